@@ -1,0 +1,331 @@
+"""Python binding of libccnn.so (the C ABI in include/ccnn.h).
+
+`FaceDetector` mirrors the reference's CompactCNNLib::FaceDetector
+(prebuild/include/CompactCNNLib.h:36-194): same members, same argument meaning and
+return codes, with numpy arrays in place of ImageData/Face buffers. Everything runs on
+the GPU through the HIP library; if the library or a HIP device is missing, loading or
+Init fails loudly -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libccnn.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+PIPELINE_CPU, PIPELINE_GPU, PIPELINE_GPU_CPU = 0, 1, 2
+DETECT_DISABLE, DETECT_SYNC, DETECT_ASYNC = 0, 1, 2
+PRECISION_DEF, PRECISION_LOW, PRECISION_NORMAL, PRECISION_HIGH, PRECISION_ULTRA = 0, 1, 2, 3, 4
+
+
+class CParam(ctypes.Structure):
+    _fields_ = [("max_image_width", _I), ("max_image_height", _I), ("min_face_height", _I),
+                ("max_face_height", _I), ("scale_factor", _F), ("min_neighbors", _I),
+                ("detect_precision", _I), ("treshold", _F * 3), ("drop_detect", _I), ("equalize", _I),
+                ("reflection", _I), ("pipeline", _I), ("detect_mode", _I), ("num_threads", _I),
+                ("models", ctypes.c_char_p * 3), ("index_output", _I * 3), ("device_info", _I),
+                ("cuda_device_id", _I), ("cl_platform_id", _I), ("cl_device_id", _I),
+                ("facial_analysis", _I)]
+
+
+class CFace(ctypes.Structure):
+    _fields_ = [("x", _I), ("y", _I), ("width", _I), ("height", _I), ("score", _F),
+                ("gender", _I), ("smile", _I), ("glasses", _I)]
+
+
+class CImage(ctypes.Structure):
+    _fields_ = [("cols", _I), ("rows", _I), ("channels", _I), ("data", _P), ("step", ctypes.c_size_t)]
+
+
+FACE_DTYPE = np.dtype([("x", np.int32), ("y", np.int32), ("width", np.int32), ("height", np.int32),
+                       ("score", np.float32), ("gender", np.int32), ("smile", np.int32), ("glasses", np.int32)])
+
+_lib = None
+
+
+def load():
+    """Loads libccnn.so (built by compactcnncascade_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libccnn.so is not built: run `python -m compactcnncascade_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    L.ccnn_param_default.argtypes = [ctypes.POINTER(CParam)]
+    L.ccnn_create.restype = _P
+    L.ccnn_destroy.argtypes = [_P]
+    L.ccnn_init.argtypes = [_P, ctypes.POINTER(CParam)]
+    L.ccnn_detect.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
+    L.ccnn_detect_device.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
+    L.ccnn_clear.argtypes = [_P]
+    L.ccnn_is_empty.argtypes = [_P]
+    L.ccnn_get_param.argtypes = [_P, ctypes.POINTER(CParam)]
+    L.ccnn_set_param.argtypes = [_P, ctypes.POINTER(CParam)]
+    L.ccnn_get_max_image_size.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]
+    L.ccnn_set_max_image_size.argtypes = [_P, _I, _I]
+    L.ccnn_get_min_object_height.argtypes = [_P]
+    L.ccnn_set_min_object_height.argtypes = [_P, _I]
+    L.ccnn_get_max_object_height.argtypes = [_P]
+    L.ccnn_set_max_object_height.argtypes = [_P, _I]
+    L.ccnn_get_num_threads.argtypes = [_P]
+    L.ccnn_set_num_threads.argtypes = [_P, _I]
+    L.ccnn_get_gray_image.argtypes = [_P, ctypes.POINTER(CImage), _I]
+    L.ccnn_stage_gray.argtypes = [ctypes.POINTER(CImage), _P]
+    L.ccnn_stage_level.argtypes = [_P, _I, _I, _I, _I, _I, _P]
+    L.ccnn_stage1_size.argtypes = [_I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]
+    L.ccnn_stage1.argtypes = [_P, _I, _I, _P]
+    L.ccnn_stage_check.argtypes = [ctypes.POINTER(CParam), _P, _I, _I, _I, _P, _P, _P, _P]
+    L.ccnn_last_trace.argtypes = [_P, ctypes.POINTER(_I), _P, _I, _P, _P]
+    L.ccnn_set_profiling.argtypes = [_P, _I]
+    L.ccnn_last_timing.argtypes = [_P, _P]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _image(img):
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    ch = 1 if img.ndim == 2 else img.shape[2]
+    return img, CImage(w, h, ch, _ptr(img), w * ch)
+
+
+class Param:
+    """FaceDetector::Param with the reference's defaults (CompactCNNLib.h:98-165)."""
+
+    def __init__(self, **kw):
+        c = CParam()
+        load().ccnn_param_default(ctypes.byref(c))
+        self.max_image_size = (c.max_image_width, c.max_image_height)
+        self.min_face_height = c.min_face_height
+        self.max_face_height = c.max_face_height
+        self.scale_factor = c.scale_factor
+        self.min_neighbors = c.min_neighbors
+        self.detect_precision = c.detect_precision
+        self.treshold = [c.treshold[0], c.treshold[1], c.treshold[2]]
+        self.drop_detect = bool(c.drop_detect)
+        self.equalize = bool(c.equalize)
+        self.reflection = bool(c.reflection)
+        self.pipeline = c.pipeline
+        self.detect_mode = c.detect_mode
+        self.num_threads = c.num_threads
+        self.index_output = [1, 0, 0]
+        self.device_info = False
+        self.cuda_device_id = 0
+        self.facial_analysis = False
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(k)
+            setattr(self, k, v)
+
+    def to_c(self):
+        c = CParam()
+        load().ccnn_param_default(ctypes.byref(c))
+        c.max_image_width, c.max_image_height = self.max_image_size
+        c.min_face_height = self.min_face_height
+        c.max_face_height = self.max_face_height
+        c.scale_factor = self.scale_factor
+        c.min_neighbors = self.min_neighbors
+        c.detect_precision = self.detect_precision
+        for i in range(3):
+            c.treshold[i] = self.treshold[i]
+            c.index_output[i] = self.index_output[i]
+        c.drop_detect = int(self.drop_detect)
+        c.equalize = int(self.equalize)
+        c.reflection = int(self.reflection)
+        c.pipeline = self.pipeline
+        c.detect_mode = self.detect_mode
+        c.num_threads = self.num_threads
+        c.device_info = int(self.device_info)
+        c.cuda_device_id = self.cuda_device_id
+        c.facial_analysis = int(self.facial_analysis)
+        return c
+
+    @classmethod
+    def from_c(cls, c):
+        p = cls()
+        p.max_image_size = (c.max_image_width, c.max_image_height)
+        p.min_face_height = c.min_face_height
+        p.max_face_height = c.max_face_height
+        p.scale_factor = c.scale_factor
+        p.min_neighbors = c.min_neighbors
+        p.detect_precision = c.detect_precision
+        p.treshold = [c.treshold[i] for i in range(3)]
+        p.drop_detect = bool(c.drop_detect)
+        p.equalize = bool(c.equalize)
+        p.reflection = bool(c.reflection)
+        p.pipeline = c.pipeline
+        p.detect_mode = c.detect_mode
+        p.num_threads = c.num_threads
+        p.index_output = [c.index_output[i] for i in range(3)]
+        p.device_info = bool(c.device_info)
+        p.cuda_device_id = c.cuda_device_id
+        p.facial_analysis = bool(c.facial_analysis)
+        return p
+
+
+class FaceDetector:
+    """CompactCNNLib::FaceDetector over libccnn.so (one detector per object)."""
+
+    def __init__(self):
+        self._L = load()
+        self._h = self._L.ccnn_create()
+        if not self._h:
+            raise MemoryError("ccnn_create failed")
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and getattr(self, "_L", None) is not None:
+            self._L.ccnn_destroy(h)
+
+    def Init(self, param):
+        c = param.to_c()
+        return self._L.ccnn_init(self._h, ctypes.byref(c))
+
+    def Detect(self, img, capacity=-1):
+        """Returns (count, faces structured array); count is -1 when not initialised."""
+        img, ci = _image(img)
+        cap = 4096
+        while True:
+            faces = np.zeros(cap, FACE_DTYPE)
+            n = self._L.ccnn_detect(self._h, _ptr(faces), cap, ctypes.byref(ci))
+            if n <= cap:
+                break
+            cap = n
+        if n < 0:
+            return n, np.zeros(0, FACE_DTYPE)
+        if capacity >= 0:
+            return n, faces[:min(n, capacity)]
+        return n, faces[:n]
+
+    def DetectDevice(self, dev_ptr, cols, rows, channels, step):
+        """Detection on a frame already resident in GPU memory (device pointer)."""
+        ci = CImage(cols, rows, channels, dev_ptr, step)
+        faces = np.zeros(4096, FACE_DTYPE)
+        n = self._L.ccnn_detect_device(self._h, _ptr(faces), 4096, ctypes.byref(ci))
+        return n, faces[:max(0, min(n, 4096))]
+
+    def Clear(self):
+        return self._L.ccnn_clear(self._h)
+
+    def isEmpty(self):
+        return bool(self._L.ccnn_is_empty(self._h))
+
+    def getParam(self):
+        c = CParam()
+        self._L.ccnn_get_param(self._h, ctypes.byref(c))
+        return Param.from_c(c)
+
+    def setParam(self, param):
+        c = param.to_c()
+        self._L.ccnn_set_param(self._h, ctypes.byref(c))
+
+    def getMaxImageSize(self):
+        w, h = _I(), _I()
+        self._L.ccnn_get_max_image_size(self._h, ctypes.byref(w), ctypes.byref(h))
+        return w.value, h.value
+
+    def setMaxImageSize(self, size):
+        self._L.ccnn_set_max_image_size(self._h, int(size[0]), int(size[1]))
+
+    def getMinObjectHeight(self):
+        return self._L.ccnn_get_min_object_height(self._h)
+
+    def setMinObjectHeight(self, v):
+        self._L.ccnn_set_min_object_height(self._h, int(v))
+
+    def getMaxObjectHeight(self):
+        return self._L.ccnn_get_max_object_height(self._h)
+
+    def setMaxObjectHeight(self, v):
+        self._L.ccnn_set_max_object_height(self._h, int(v))
+
+    def getNumThreads(self):
+        return self._L.ccnn_get_num_threads(self._h)
+
+    def setNumThreads(self, n):
+        self._L.ccnn_set_num_threads(self._h, int(n))
+
+    def getGrayImage32F(self, pipeline=PIPELINE_CPU):
+        """Host copy (PIPELINE_CPU) of the last frame's FP32 gray image, or None."""
+        ci = CImage()
+        if self._L.ccnn_get_gray_image(self._h, ctypes.byref(ci), pipeline) != 0:
+            return None
+        if pipeline != PIPELINE_CPU:
+            return ci
+        buf = (ctypes.c_float * (ci.rows * ci.step)).from_address(ci.data)
+        return np.frombuffer(buf, np.float32).reshape(ci.rows, ci.step)[:, :ci.cols].copy()
+
+    # -- diagnostics -------------------------------------------------------------
+    def set_profiling(self, on=True):
+        self._L.ccnn_set_profiling(self._h, int(on))
+
+    def last_timing(self):
+        ms = np.zeros(5, np.float32)
+        self._L.ccnn_last_timing(self._h, _ptr(ms))
+        return dict(gray=float(ms[0]), stage1=float(ms[1]), compact=float(ms[2]), check=float(ms[3]),
+                    total=float(ms[4]))
+
+    def last_trace(self):
+        """Levels run and candidate windows of the last Detect, with their check results."""
+        nl = _I()
+        n = self._L.ccnn_last_trace(self._h, ctypes.byref(nl), None, 0, None, None)
+        levels = np.zeros((max(1, nl.value), 6), np.int32)
+        cand = np.zeros((max(1, n), 4), np.float32)
+        res = np.zeros((max(1, n), 8), np.float32)
+        self._L.ccnn_last_trace(self._h, ctypes.byref(nl), _ptr(levels), n, _ptr(cand), _ptr(res))
+        return dict(levels=levels[:nl.value], cand=cand[:n], res=res[:n])
+
+
+# -- stage-level entry points --------------------------------------------------------
+def stage_gray(img):
+    img, ci = _image(img)
+    out = np.zeros((ci.rows, ci.cols), np.float32)
+    if load().ccnn_stage_gray(ctypes.byref(ci), _ptr(out)) != 0:
+        raise RuntimeError("ccnn_stage_gray failed")
+    return out
+
+
+def stage_level(gray, lw, lh, mode):
+    gray = np.ascontiguousarray(gray, np.float32)
+    gh, gw = gray.shape
+    out = np.zeros((lh, lw), np.float32)
+    if load().ccnn_stage_level(_ptr(gray), gw, gh, lw, lh, mode, _ptr(out)) != 0:
+        raise RuntimeError("ccnn_stage_level failed")
+    return out
+
+
+def stage1(level):
+    level = np.ascontiguousarray(level, np.float32)
+    lh, lw = level.shape
+    rw, rh = _I(), _I()
+    L = load()
+    if L.ccnn_stage1_size(lw, lh, ctypes.byref(rw), ctypes.byref(rh)) != 0:
+        raise ValueError("level too small")
+    out = np.zeros((rh.value, rw.value), np.float32)
+    if L.ccnn_stage1(_ptr(level), lw, lh, _ptr(out)) != 0:
+        raise RuntimeError("ccnn_stage1 failed")
+    return out
+
+
+def stage_check(param, gray, scales, xs, ys):
+    gray = np.ascontiguousarray(gray, np.float32)
+    gh, gw = gray.shape
+    scales = np.ascontiguousarray(scales, np.float32)
+    xs = np.ascontiguousarray(xs, np.int32)
+    ys = np.ascontiguousarray(ys, np.int32)
+    n = len(scales)
+    res = np.zeros((max(1, n), 8), np.float32)
+    c = param.to_c()
+    if load().ccnn_stage_check(ctypes.byref(c), _ptr(gray), gw, gh, n, _ptr(scales), _ptr(xs), _ptr(ys),
+                               _ptr(res)) != 0:
+        raise RuntimeError("ccnn_stage_check failed")
+    return res[:n]

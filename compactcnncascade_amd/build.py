@@ -1,0 +1,75 @@
+"""Builds libccnn.so (HIP kernels for gfx950 + C++ host side + C ABI) in-tree.
+
+    python -m compactcnncascade_amd.build
+
+The shipped model files are embedded into the library as byte arrays, as the
+reference embeds them (serialized_models.h), so the library has no run-time file
+dependency.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libccnn.so")
+MODELS = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin"]
+
+SOURCES = ["ccnn_kernels.hip", "ccnn_detector.cpp", "ccnn_api.cpp", "ccnn_stages.cpp"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CCNN_OFFLOAD_ARCH", "gfx950")
+
+
+def gen_models(path):
+    lines = ["// generated from compactcnncascade_amd/models by build.py", "#include <cstddef>",
+             "namespace ccnn {"]
+    for i, name in enumerate(MODELS):
+        with open(os.path.join(PKG, "models", name), "rb") as f:
+            data = f.read()
+        body = ",".join(str(b) for b in data)
+        lines.append("static const unsigned char k_model%d[%d] = {%s};" % (i, len(data), body))
+    lines.append("void embedded_model(int stage, const unsigned char** bytes, size_t* n)")
+    lines.append("{")
+    for i in range(3):
+        lines.append("    if (stage == %d) { *bytes = k_model%d; *n = sizeof(k_model%d); return; }" % (i, i, i))
+    lines.append("    *bytes = nullptr; *n = 0;")
+    lines.append("}")
+    lines.append("}  // namespace ccnn")
+    text = "\n".join(lines) + "\n"
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as f:
+            f.write(text)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    models_cpp = os.path.join(BUILD, "ccnn_models.cpp")
+    gen_models(models_cpp)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES] + [models_cpp]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+           # exact path: no a*b+c contraction anywhere (the reference rounds each op)
+           "-ffp-contract=off",
+           # packed FP32 instructions stay off on this target
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument",
+           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+           "-o", tmp] + srcs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if out.strip() and verbose:
+        print(out)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed (%d):\n%s" % (r.returncode, out))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,147 @@
+// Host side of the MI355X detection path: model loading, pyramid plan, device buffers,
+// kernel dispatch and the ordered replay of the reference's order-dependent decisions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ccnn.h"
+#include "ccnn_internal.h"
+
+namespace ccnn {
+
+// kernel launchers (ccnn_kernels.hip)
+hipError_t launch_gray(const uint8_t* img, int w, int h, int ch, int step, float* gray, int gp, hipStream_t st);
+hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8_t* dst, int dw, int dh,
+                            hipStream_t st);
+hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
+hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
+                         int ntiles, const float* W1, float* resp, hipStream_t st);
+hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
+                          int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st);
+hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
+                        const int* meta, const CheckParams& P, const float* W2, const float* W3,
+                        CheckResult* results, int nblocks, hipStream_t st);
+
+// A model of the cascade in the reference's binary format, repacked in NetShape layout.
+struct NetWeights {
+    std::vector<float> w;
+    int min_w = 0, min_h = 0;
+};
+// Parses format v1.0-1.1 (cnn_simd_cntk.cpp:42-404) and checks the shape against the
+// stage's prototype. Returns 0 or -1 (message in err).
+int load_model(const unsigned char* bytes, size_t n, int stage, int index_output, NetWeights& out, std::string& err);
+// Embedded copies of the shipped models (cnn4face{1,2,3}_cntk.bin).
+void embedded_model(int stage, const unsigned char** bytes, size_t* n);
+
+// Effective detector settings (FaceDetector::Init's conversion, CompactCNNLibAPI_v2.cpp:100-138,
+// and CNNDetector::Init, cnn_detector_v3.cpp:922-964).
+struct Settings {
+    int type_check = 1, min_num_detect = 1, adapt = 0, drop_detect = 1, merger = 1, double_check = 1;
+    int equalize = 1, reflection = 1, detect_mode = 2, min_neighbors = 2;
+    float t1 = -1.5f, t2 = -1.25f, t3 = -1.0f;
+};
+Settings effective_settings(const ccnn_param& p);
+
+// InitScales (cnn_detector_v3.cpp:289-346). Returns the scales (largest first); pre_scale
+// receives the gray_image_only pre-resize factor (0 if none). Empty on failure.
+std::vector<float> init_scales(const ccnn_param& p, int pattern_h, bool gray_only, float* pre_scale,
+                               float* scale_factor_used);
+
+struct Rect {
+    int x = 0, y = 0, w = 0, h = 0, x2 = 0, y2 = 0;
+    Rect() = default;
+    Rect(int x_, int y_, int w_, int h_) : x(x_), y(y_), w(w_), h(h_), x2(x_ + w_), y2(y_ + h_) {}
+};
+struct Det {
+    Rect r;
+    float score = 0.f, scale = 0.f;
+    int knn = 0, num_detect = 0;
+    bool checked = false;
+};
+// Merger (cnn_detector_v3.cpp:2814-2936)
+void merger(std::vector<Det>& out, std::vector<Det>& in, float threshold, bool del);
+// Post-processing of Detect (cnn_detector_v3.cpp:2745-2781): reverse, sort by scale,
+// Merger 0.5, min_num_detect filter, Merger 0.2 with deletion.
+std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s);
+
+class Detector {
+public:
+    Detector();
+    ~Detector();
+
+    int init(const ccnn_param& p);
+    int detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces);
+    int clear();
+    bool empty() const { return empty_; }
+
+    ccnn_param param;          // the API-level parameters (getParam)
+    int gray_image(ccnn_image* img, int pipeline);
+    void set_profiling(bool on) { profiling_ = on; }
+    int last_timing(float* ms) const;
+    int last_trace(int* n_levels, int* levels, int cap, float* cand, float* res) const;
+
+    // stage-level access (tests)
+    int stage_check(const float* gray, int gw, int gh, int n, const float* scale, const int* x, const int* y,
+                    float* res);
+
+private:
+    int init_device();
+    int build();                                  // Init chain after parameter changes
+    int ensure_buffers(int w, int h);
+    int plan_frame(int gw, int gh);
+    int run_frame(const uint8_t* d_src, int w, int h, int ch, size_t step, std::vector<ccnn_face>& faces);
+    void release();
+
+    bool empty_ = true;
+    bool built_ = false;
+    bool gray_only_ = true;        // AdvancedParam::gray_image_only as forced by the API
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    Settings set_;
+    NetWeights net_[3];
+    float* d_w_[3] = {nullptr, nullptr, nullptr};
+    std::vector<float> scales_;
+    float pre_scale_ = 0.f;
+    int ext_w_ = 47, ext_h_ = 51;
+
+    // buffers sized for max_image_size
+    int cap_w_ = 0, cap_h_ = 0;
+    uint8_t* d_img_ = nullptr;  size_t img_bytes_ = 0;
+    uint8_t* d_pre_ = nullptr;
+    float* d_gray_ = nullptr;   int gpitch_ = 0;
+    float* d_resp_ = nullptr;   size_t resp_cap_ = 0;
+    LevelDesc* d_levels_ = nullptr;
+    int* d_counts_ = nullptr;   int* d_offsets_ = nullptr; size_t chunk_cap_ = 0;
+    int* d_meta_ = nullptr;
+    Candidate* d_cands_ = nullptr;
+    CheckResult* d_results_ = nullptr;
+    size_t cand_cap_ = 0;
+    int* h_meta_ = nullptr;
+    Candidate* h_cands_ = nullptr;
+    CheckResult* h_results_ = nullptr;
+    size_t h_cap_ = 0;
+    std::vector<float> h_gray_;
+
+    // plan of the current frame size
+    int plan_gw_ = -1, plan_gh_ = -1;
+    std::vector<LevelDesc> levels_;
+    int ntiles_ = 0, nchunks_ = 0;
+    size_t resp_total_ = 0;
+    int last_gw_ = 0, last_gh_ = 0;
+
+    // last frame record
+    int last_count_ = 0;
+    std::vector<Candidate> last_cands_;
+    std::vector<CheckResult> last_results_;
+    std::vector<LevelDesc> last_levels_;
+
+    bool profiling_ = false;
+    hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float last_ms_[5] = {0, 0, 0, 0, 0};
+    int check_blocks_ = 2048;
+};
+
+}  // namespace ccnn

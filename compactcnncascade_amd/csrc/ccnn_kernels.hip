@@ -1,0 +1,747 @@
+// gfx950 kernels of the compact-CNN-cascade detection path.
+//
+// Exact path: every floating-point expression is evaluated in the order the reference's
+// CPU build evaluates it (built with -ffp-contract=off, so a*b + c stays a rounded
+// multiply followed by a rounded add), which makes gray conversion, the pyramid, the
+// response maps, the crops and the stage-2/3 responses bit-identical to the reference
+// CPU path. References are to Bkmz21/CompactCNNCascade src/CNNObjectDetector/.
+//
+//   k_gray         u8 BGR/BGRA/gray -> FP32 gray             (image_proc.cpp:99-102,195-201,286-295)
+//   k_level        one pyramid level, materialised (tests)   (image_resize.cpp:46-122,328-541)
+//   k_stage1       fused pyramid sample + L1..OL per 16x16   (cnn_simd_cntk.cpp:640-963,
+//                  response tile, all levels in one launch    cnnpp_cplusplus.cpp)
+//   k_count / k_scan / k_scatter
+//                  threshold + order-preserving compaction   (cnn_detector_v3.cpp:1918-1947)
+//                  of candidate windows (levels in check order, raster order inside a level)
+//   k_check        per-candidate crop, NN resize to 49x51,   (cnn_detector_v3.cpp:1485-1818,
+//                  u8, equalize, stage 2 + stage 3, retry     image_proc.cpp:357-368,720-800)
+//                  crop; one workgroup per candidate
+
+#include <hip/hip_runtime.h>
+#include "ccnn_internal.h"
+
+namespace ccnn {
+
+__device__ __forceinline__ float rtanh(float c)
+{
+    // sgn * (1 - 1/(1 + |c| + c^2 + a c^4)), cnnpp_cplusplus.cpp:662-666, with the
+    // association of the reference's -ffast-math build: ((|c| + c*c) + a*c*c*c*c) + 1.
+    const float a = 1.41645f;
+    float sgn = 1.f;
+    if (c < 0.f) sgn = -1.f;
+    const float den = ((fabsf(c) + c * c) + a * c * c * c * c) + 1.f;
+    return (1.f - 1.f / den) * sgn;
+}
+
+__device__ __forceinline__ float lrelu_bn(float conv, const float* __restrict__ p, int m, int M)
+{
+    // w1*c + w2*max(0,c) + bn_b with c = conv + conv_b (cnnpp_cplusplus.cpp:604-605,
+    // 630-631), associated as the reference's build evaluates it.
+    const float c = conv + p[m];
+    return (p[2 * M + m] * fmaxf(0.f, c) + p[3 * M + m]) + p[M + m] * c;
+}
+
+// Sample of pyramid level L at (lx, ly), from the FP32 gray image
+// (ImageResizer::preprocessing + Nearest/BilinearInterpolation, scalar tails).
+__device__ __forceinline__ float level_px(const float* __restrict__ g, int gp, int gw, int gh,
+                                          const LevelDesc& L, int lx, int ly)
+{
+    if (L.mode == 2) return g[(size_t)ly * gp + lx];
+    const uint32_t y = (uint32_t)ly * L.scly;
+    uint32_t py = y >> 12;
+    const uint32_t x = (uint32_t)lx * L.sclx;
+    uint32_t px = x >> 12;
+    if (L.mode == 0) {
+        py = min(py, (uint32_t)(gh - 1));
+        px = min(px, (uint32_t)(gw - 2));
+        return g[(size_t)py * gp + px];
+    }
+    float fy = (float)(y - (py << 12));
+    float cy = 4096.f - fy;
+    fy = fy / 16777216.f;
+    cy = cy / 16777216.f;
+    py = min(py, (uint32_t)(gh - 1));
+    if (py + 1 >= (uint32_t)gh) py--;
+    const float fx = (float)(x - (px << 12));
+    const float cx = 4096.f - fx;
+    px = min(px, (uint32_t)(gw - 2));
+    const float* r0 = g + (size_t)py * gp + px;
+    const float* r1 = r0 + gp;
+    const float p0 = r0[0], p1 = r0[1], p2 = r1[0], p3 = r1[1];
+    return (p0 * cx + p1 * fx) * cy + (p2 * cx + p3 * fx) * fy;
+}
+
+// ---------------------------------------------------------------------------------
+// Gray conversion
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, int w, int h, int ch, int step,
+                                              float* __restrict__ gray, int gp)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= w || y >= h) return;
+    const uint8_t* s = img + (size_t)y * step + (size_t)x * ch;
+    float v;
+    if (ch == 1) {
+        v = (float)s[0];
+    } else {
+        const float B = (float)s[0], G = (float)s[1], R = (float)s[2];
+        v = 0.114f * B + 0.587f * G + 0.299f * R;
+    }
+    gray[(size_t)y * gp + x] = v;
+}
+
+// 8-bit bilinear resize (ImageResizer::BilinearInterpolation for Image_8u, scalar tail,
+// image_resize.cpp:198-326): the gray_image_only pre-resize of a 1-channel frame.
+__global__ void __launch_bounds__(256) k_resize_u8(const uint8_t* __restrict__ src, int sw, int sh, int sstep,
+                                                   uint8_t* __restrict__ dst, int dw, int dh, uint32_t sclx,
+                                                   uint32_t scly)
+{
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    if (ix >= dw || iy >= dh) return;
+    const uint32_t y = (uint32_t)iy * scly;
+    uint32_t py = y >> 12;
+    float fy = (float)(y - (py << 12));
+    float cy = 4096.f - fy;
+    fy = fy / 16777216.f;
+    cy = cy / 16777216.f;
+    py = min(py, (uint32_t)(sh - 1));
+    if (py + 1 >= (uint32_t)sh) py--;
+    const uint32_t x = (uint32_t)ix * sclx;
+    uint32_t px = x >> 12;
+    const float fx = (float)(x - (px << 12));
+    const float cx = 4096.f - fx;
+    px = min(px, (uint32_t)(sw - 2));
+    const uint8_t* r0 = src + (size_t)py * sstep + px;
+    const uint8_t* r1 = r0 + sstep;
+    const float p0 = r0[0], p1 = r0[1], p2 = r1[0], p3 = r1[1];
+    const float v = (p0 * cx + p1 * fx) * cy + (p2 * cx + p3 * fx) * fy;
+    dst[(size_t)iy * dw + ix] = (uint8_t)(uint32_t)v;
+}
+
+// One level image (tests and getGrayImage-style inspection; the detection path samples
+// levels inside k_stage1 instead of storing them).
+__global__ void __launch_bounds__(256) k_level(const float* __restrict__ g, int gp, int gw, int gh, LevelDesc L,
+                                               float* __restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= L.lw || y >= L.lh) return;
+    out[(size_t)y * L.lw + x] = level_px(g, gp, gw, gh, L, x, y);
+}
+
+// ---------------------------------------------------------------------------------
+// Stage 1: fused level sampling + L1 (conv 4x4, act, pool) + sums + L2 (conv 3x3,
+// act, pool) + sums + L3 (conv 4w x 5h, act) + hidden layer + output layer.
+// One workgroup computes a 16x16 tile of one level's response map.
+// ---------------------------------------------------------------------------------
+namespace s1 {
+using N = Stage1;
+constexpr int TX = S1_TX, TY = S1_TY;
+constexpr int IW = 4 * TX + 19, IH = 4 * TY + 23;     // level input window
+constexpr int P1W = 2 * TX + 8, P1H = 2 * TY + 10;    // pooled L1 / first sums
+constexpr int P2W = TX + 3, P2H = TY + 4;             // pooled L2 / second sums
+constexpr int P1N = P1W * P1H, P2N = P2W * P2H;
+}  // namespace s1
+
+__device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n, int t, bool chunks)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int off = chunks ? L[mid].chunk_off : L[mid].tile_off;
+        if (off <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__ g, int gp, int gw, int gh,
+                                                       const LevelDesc* __restrict__ levels, int nlevels,
+                                                       const float* __restrict__ W, float* __restrict__ resp)
+{
+    using namespace s1;
+    __shared__ float s_in[IH * IW];
+    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later reused for the L2 sums
+    __shared__ float s_p2[N::M2 * P2N];
+
+    const int tid = threadIdx.x;
+    const int li = find_level(levels, nlevels, blockIdx.x, false);
+    const LevelDesc L = levels[li];
+    const int t = blockIdx.x - L.tile_off;
+    const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
+    const int x0 = 4 * ox0, y0 = 4 * oy0;
+
+    // input window (clamped at the level border; clamped samples only feed outputs
+    // outside the response map, which are not stored)
+    for (int i = tid; i < IW * IH; i += S1_THREADS) {
+        const int iy = i / IW, ix = i - iy * IW;
+        const int lx = min(x0 + ix, L.lw - 1), ly = min(y0 + iy, L.lh - 1);
+        s_in[i] = level_px(g, gp, gw, gh, L, lx, ly);
+    }
+    __syncthreads();
+
+    // L1: conv_4x4 chains + lrelu_bn_max for every map, then the sums feeding L2
+    // (add / add2, cnnpp_cplusplus.cpp:916-950; wiring cnn_simd_cntk.cpp:705-725)
+    const float* k1 = W + N::oK1;
+    const float* b1 = W + N::oB1;
+    for (int p = tid; p < P1N; p += S1_THREADS) {
+        const int py = p / P1W, px = p - py * P1W;
+        float win[5][5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) win[r][c] = s_in[(2 * py + r) * IW + 2 * px + c];
+        float pm[N::M1];
+#pragma unroll
+        for (int m = 0; m < N::M1; ++m) {
+            const float* k = k1 + m * 16;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 16; ++j) acc = acc + win[dy + (j >> 2)][dx + (j & 3)] * k[j];
+                a[q] = lrelu_bn(acc, b1, m, N::M1);
+            }
+            pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+        s_s1[0 * P1N + p] = pm[0] + pm[1];
+#pragma unroll
+        for (int i = 1; i < N::M1 - 1; ++i) s_s1[i * P1N + p] = pm[i - 1] + pm[i] + pm[i + 1];
+        s_s1[(N::M1 - 1) * P1N + p] = pm[N::M1 - 2] + pm[N::M1 - 1];
+    }
+    __syncthreads();
+
+    // L2: two conv_3x3 chains per sum + lrelu_bn_max
+    const float* k2 = W + N::oK2;
+    const float* b2 = W + N::oB2;
+    for (int it = tid; it < N::M1 * P2N; it += S1_THREADS) {
+        const int i = it / P2N, p = it - i * P2N;
+        const int qy = p / P2W, qx = p - qy * P2W;
+        const float* src = s_s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
+        float win[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int m = 2 * i + half;
+            const float* k = k2 + m * 9;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
+                a[q] = lrelu_bn(acc, b2, m, N::M2);
+            }
+            s_p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+    }
+    __syncthreads();
+    float* s_s2 = s_s1;   // L2 sums overwrite the (consumed) L1 sums
+    for (int p = tid; p < P2N; p += S1_THREADS) {
+        float v[N::M2];
+#pragma unroll
+        for (int m = 0; m < N::M2; ++m) v[m] = s_p2[m * P2N + p];
+        s_s2[p] = v[0] + v[1];
+#pragma unroll
+        for (int i = 1; i < N::M2 - 1; ++i) s_s2[i * P2N + p] = v[i - 1] + v[i] + v[i + 1];
+        s_s2[(N::M2 - 1) * P2N + p] = v[N::M2 - 2] + v[N::M2 - 1];
+    }
+    __syncthreads();
+
+    // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
+    // then the hidden and output layers in registers, one response value per thread.
+    const int ox = tid % TX, oy = tid / TX;
+    const float* k3 = W + N::oK3;
+    const float* b3 = W + N::oB3;
+    float a3[N::M3];
+#pragma unroll
+    for (int i = 0; i < N::M2; ++i) {
+        const float* src = s_s2 + i * P2N + oy * P2W + ox;
+        float win[N::K3H][N::K3W];
+#pragma unroll
+        for (int r = 0; r < N::K3H; ++r)
+#pragma unroll
+            for (int c = 0; c < N::K3W; ++c) win[r][c] = src[r * P2W + c];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int m = 2 * i + half;
+            const float* k = k3 + m * N::K3W * N::K3H;
+            float d = 0.f;
+#pragma unroll
+            for (int r = 0; r < N::K3H; ++r) {
+                float rs = win[r][0] * k[r * N::K3W];
+#pragma unroll
+                for (int c = 1; c < N::K3W; ++c) rs = rs + win[r][c] * k[r * N::K3W + c];
+                d = (r == 0) ? rs : d + rs;
+            }
+            a3[m] = lrelu_bn(d, b3, m, N::M3);
+        }
+    }
+    // hidden layer: mulCN_add_tanhW (cnnpp_cplusplus.cpp:646-672, loop cnn_simd_cntk.cpp:874-883)
+    float hv[N::HL];
+#pragma unroll
+    for (int gi = 0; gi < N::HLG; ++gi)
+#pragma unroll
+        for (int tt = 0; tt < N::HLS; ++tt) {
+            const int u = N::HLS * gi + tt;
+            float c = W[N::oHB + u];
+#pragma unroll
+            for (int j = 0; j < N::HLN; ++j) c = c + a3[gi + j] * W[N::oHW + u * N::HLN + j];
+            c = c * W[N::oHT + gi];
+            c = rtanh(c);
+            c = (c + 1.f) * 0.5f;
+            hv[u] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+        }
+    // output layer: mulC / mulC1_add / tanhW (cnnpp_cplusplus.cpp:952-984, 673-690)
+    float o = 0.f;
+#pragma unroll
+    for (int gi = 0; gi < N::HLG; ++gi)
+#pragma unroll
+        for (int tt = 0; tt < N::HLS; ++tt) {
+            const float wv = W[N::oOW + tt * N::HLG + gi];
+            if (gi == 0 && tt == 0) o = hv[0] * wv;
+            else o = hv[N::HLS * gi + tt] * wv + o;
+        }
+    float c = o + W[N::oOB];
+    c = c * W[N::oOB + 1];
+    c = rtanh(c);
+    const float af = W[N::oOB + 2];
+    const float out = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
+    const int gx = ox0 + ox, gy = oy0 + oy;
+    if (gx < L.rw && gy < L.rh) resp[L.resp_off + gy * L.rw + gx] = out;
+}
+
+// ---------------------------------------------------------------------------------
+// Candidate compaction: responses > t1, levels in check order (the level table is in
+// that order), raster order inside a level.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int block_excl_scan_256(int v, int* s_tmp, int* total)
+{
+    // exclusive scan of one int per thread over a 256-thread block
+    const int tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a = tid >= off ? s_tmp[tid - off] : 0;
+        __syncthreads();
+        s_tmp[tid] += a;
+        __syncthreads();
+    }
+    const int incl = s_tmp[tid];
+    *total = s_tmp[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(256) k_count(const float* __restrict__ resp, const LevelDesc* __restrict__ levels,
+                                               int nlevels, float t1, int* __restrict__ counts)
+{
+    __shared__ int s_tmp[256];
+    const int c = blockIdx.x;
+    const int li = find_level(levels, nlevels, c, true);
+    const LevelDesc L = levels[li];
+    const int n = L.rw * L.rh;
+    const int base = (c - L.chunk_off) * CMP_CHUNK;
+    int cnt = 0;
+    for (int k = 0; k < CMP_CHUNK / 256; ++k) {
+        const int e = base + threadIdx.x * (CMP_CHUNK / 256) + k;
+        if (e < n && resp[L.resp_off + e] > t1) cnt++;
+    }
+    int total;
+    block_excl_scan_256(cnt, s_tmp, &total);
+    if (threadIdx.x == 0) counts[c] = total;
+}
+
+// Single workgroup: exclusive scan over the chunk counts; total written to meta[0].
+__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int nchunks, int* __restrict__ offsets,
+                                               int* __restrict__ meta)
+{
+    __shared__ int s_tmp[1024];
+    const int tid = threadIdx.x;
+    const int per = (nchunks + 1023) / 1024;
+    const int b = tid * per;
+    int sum = 0;
+    for (int k = 0; k < per; ++k)
+        if (b + k < nchunks) sum += counts[b + k];
+    s_tmp[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int a = tid >= off ? s_tmp[tid - off] : 0;
+        __syncthreads();
+        s_tmp[tid] += a;
+        __syncthreads();
+    }
+    int run = s_tmp[tid] - sum;
+    for (int k = 0; k < per; ++k)
+        if (b + k < nchunks) { offsets[b + k] = run; run += counts[b + k]; }
+    if (tid == 1023) meta[0] = s_tmp[1023];
+}
+
+__global__ void __launch_bounds__(256) k_scatter(const float* __restrict__ resp, const LevelDesc* __restrict__ levels,
+                                                 int nlevels, float t1, const int* __restrict__ offsets,
+                                                 Candidate* __restrict__ cands)
+{
+    __shared__ int s_tmp[256];
+    const int c = blockIdx.x;
+    const int li = find_level(levels, nlevels, c, true);
+    const LevelDesc L = levels[li];
+    const int n = L.rw * L.rh;
+    const int base = (c - L.chunk_off) * CMP_CHUNK;
+    constexpr int PER = CMP_CHUNK / 256;
+    float v[PER];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = base + threadIdx.x * PER + k;
+        v[k] = e < n ? resp[L.resp_off + e] : 0.f;
+        if (e < n && v[k] > t1) cnt++;
+    }
+    int total;
+    int pos = offsets[c] + block_excl_scan_256(cnt, s_tmp, &total);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = base + threadIdx.x * PER + k;
+        if (e < n && v[k] > t1) {
+            Candidate cd;
+            cd.level = li;
+            cd.y = e / L.rw;
+            cd.x = e - cd.y * L.rw;
+            cd.score = v[k];
+            cands[pos++] = cd;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Stages 2/3: one workgroup evaluates one candidate's whole check
+// (CPUCheckDetect without the order-dependent drop logic, which the host replays).
+// ---------------------------------------------------------------------------------
+namespace chk {
+constexpr int CW = 49, CH = 51;          // crop after the NN resize (ext pattern + x offset)
+constexpr int IW = 47, IH = 51;          // stage-2/3 input
+constexpr int C1W = IW - 3, C1H = IH - 3;          // 44 x 48
+constexpr int P1W = C1W / 2, P1H = C1H / 2;        // 22 x 24
+constexpr int P1N = P1W * P1H;
+constexpr int P2W = (P1W - 2) / 2, P2H = (P1H - 2) / 2;   // 10 x 11
+constexpr int P2N = P2W * P2H;
+constexpr int RW = P2W - 7 + 1, RH = P2H - 8 + 1;  // 4 x 4
+constexpr int RN = RW * RH;
+constexpr int HLMAX = 84;
+}  // namespace chk
+
+struct ChkSmem {
+    float in[chk::IH * chk::IW];
+    float s1[6 * chk::P1N];
+    float p2[12 * chk::P2N];
+    float a3[24 * chk::RN];
+    float h[chk::HLMAX * chk::RN];
+    float out[chk::RN];
+    int hist[256];
+    int lut[256];
+    uint8_t u8[chk::CH * chk::CW + 3];
+};
+
+template <class N>
+__device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
+{
+    using namespace chk;
+    const int tid = threadIdx.x;
+    // L1 + sums
+    for (int p = tid; p < P1N; p += CHECK_THREADS) {
+        const int py = p / P1W, px = p - py * P1W;
+        float win[5][5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) win[r][c] = s.in[(2 * py + r) * IW + 2 * px + c];
+        float pm[N::M1];
+#pragma unroll
+        for (int m = 0; m < N::M1; ++m) {
+            const float* k = W + N::oK1 + m * 16;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 16; ++j) acc = acc + win[dy + (j >> 2)][dx + (j & 3)] * k[j];
+                a[q] = lrelu_bn(acc, W + N::oB1, m, N::M1);
+            }
+            pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+        s.s1[p] = pm[0] + pm[1];
+#pragma unroll
+        for (int i = 1; i < N::M1 - 1; ++i) s.s1[i * P1N + p] = pm[i - 1] + pm[i] + pm[i + 1];
+        s.s1[(N::M1 - 1) * P1N + p] = pm[N::M1 - 2] + pm[N::M1 - 1];
+    }
+    __syncthreads();
+    // L2
+    for (int it = tid; it < N::M1 * P2N; it += CHECK_THREADS) {
+        const int i = it / P2N, p = it - i * P2N;
+        const int qy = p / P2W, qx = p - qy * P2W;
+        const float* src = s.s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
+        float win[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int m = 2 * i + half;
+            const float* k = W + N::oK2 + m * 9;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
+                a[q] = lrelu_bn(acc, W + N::oB2, m, N::M2);
+            }
+            s.p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+    }
+    __syncthreads();
+    float* s2 = s.s1;
+    for (int p = tid; p < P2N; p += CHECK_THREADS) {
+        float v[N::M2];
+#pragma unroll
+        for (int m = 0; m < N::M2; ++m) v[m] = s.p2[m * P2N + p];
+        s2[p] = v[0] + v[1];
+#pragma unroll
+        for (int i = 1; i < N::M2 - 1; ++i) s2[i * P2N + p] = v[i - 1] + v[i] + v[i + 1];
+        s2[(N::M2 - 1) * P2N + p] = v[N::M2 - 2] + v[N::M2 - 1];
+    }
+    __syncthreads();
+    // L3: conv_8x7 single chain (cnnpp_cplusplus.cpp:292-321) + lrelu_bn
+    for (int it = tid; it < N::M2 * RN; it += CHECK_THREADS) {
+        const int i = it / RN, p = it - i * RN;
+        const int y = p / RW, x = p - y * RW;
+        const float* src = s2 + i * P2N + y * P2W + x;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int m = 2 * i + half;
+            const float* k = W + N::oK3 + m * N::K3W * N::K3H;
+            float acc = src[0] * k[0];
+#pragma unroll
+            for (int j = 1; j < N::K3W * N::K3H; ++j)
+                acc = acc + src[(j / N::K3W) * P2W + (j % N::K3W)] * k[j];
+            s.a3[m * RN + p] = lrelu_bn(acc, W + N::oB3, m, N::M3);
+        }
+    }
+    __syncthreads();
+    // hidden layer
+    for (int it = tid; it < N::HL * RN; it += CHECK_THREADS) {
+        const int u = it / RN, p = it - u * RN;
+        const int gi = u / N::HLS, tt = u - gi * N::HLS;
+        float c = W[N::oHB + u];
+#pragma unroll
+        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[(gi + j) * RN + p] * W[N::oHW + u * N::HLN + j];
+        c = c * W[N::oHT + gi];
+        c = rtanh(c);
+        c = (c + 1.f) * 0.5f;
+        s.h[u * RN + p] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+    }
+    __syncthreads();
+    // output layer
+    if (tid < RN) {
+        float o = 0.f;
+        for (int gi = 0; gi < N::HLG; ++gi)
+            for (int tt = 0; tt < N::HLS; ++tt) {
+                const float wv = W[N::oOW + tt * N::HLG + gi];
+                if (gi == 0 && tt == 0) o = s.h[tid] * wv;
+                else o = s.h[(N::HLS * gi + tt) * RN + tid] * wv + o;
+            }
+        float c = o + W[N::oOB];
+        c = c * W[N::oOB + 1];
+        c = rtanh(c);
+        const float af = W[N::oOB + 2];
+        s.out[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict__ g, int gp, int gw, int gh,
+                                                         const LevelDesc* __restrict__ levels,
+                                                         const Candidate* __restrict__ cands,
+                                                         const int* __restrict__ meta, CheckParams P,
+                                                         const float* __restrict__ W2, const float* __restrict__ W3,
+                                                         CheckResult* __restrict__ results)
+{
+    using namespace chk;
+    __shared__ ChkSmem s;
+    const int tid = threadIdx.x;
+    const int total = meta[0];
+    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+        const Candidate cd = cands[ci];
+        const float scale = levels[cd.level].scale;
+        const int pxp = cd.x * 4, pyp = cd.y * 4;          // shift_pattern = 4
+        const float inv = 1.f / scale;
+        const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
+        CheckResult res;
+        for (int mod = 0; mod < 2; ++mod) {
+            // crop geometry (cnn_detector_v3.cpp:1491-1556)
+            int x = (int)((float)(pxp - 6) * inv);
+            int y = (int)((float)(pyp - 5) * inv);
+            int width = (int)(36.f * inv);
+            int height = (int)(38.f * inv);
+            if (mod == 1) {
+                // x += w*0.15/2; w -= w*0.15 (folded by the reference's build into w * 0.85f)
+                x = (int)((float)x + (float)width * 0.15f / 2.f);
+                width = (int)((float)width * 0.85f);
+                y = (int)((float)y + (float)height * 0.15f / 2.f);
+                height = (int)((float)height * 0.85f);
+            }
+            const int rx = min(max(x, 0), gw);
+            const int ry = min(max(y, 0), gh);
+            const int rcols = min(max(x + width, 0), gw) - rx;
+            const int rrows = min(max(y + height, 0), gh) - ry;
+            // NN resize to 49x51 (or copy at that size) + FloatToUChar (+0.5, truncate)
+            const bool copy = (rcols == CW && rrows == CH);
+            const uint32_t sclx = ((uint32_t)rcols << 12) / CW + 1;
+            const uint32_t scly = ((uint32_t)rrows << 12) / CH + 1;
+            for (int i = tid; i < 256; i += CHECK_THREADS) s.hist[i] = 0;
+            __syncthreads();
+            for (int i = tid; i < CW * CH; i += CHECK_THREADS) {
+                const int iy = i / CW, ix = i - iy * CW;
+                uint32_t px = ix, py = iy;
+                if (!copy) {
+                    px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(rcols - 2));
+                    py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(rrows - 1));
+                }
+                const float v = g[(size_t)(ry + py) * gp + rx + px];
+                const uint32_t u = (uint32_t)(v + 0.5f);
+                s.u8[i] = (uint8_t)u;
+                if (P.equalize) atomicAdd(&s.hist[u & 255], 1);
+            }
+            __syncthreads();
+            if (P.equalize) {
+                // equalizeImage (image_proc.cpp:720-800, scalar branch)
+                if (tid == 0) {
+                    int acc = 0;
+                    for (int k = 0; k < 256; ++k) { acc += s.hist[k]; s.lut[k] = acc; }
+                }
+                __syncthreads();
+                const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
+                const bool apply = bmin != bmax;
+                if (apply) {
+                    const float delt = 255.f / (bmax - bmin);
+                    const int cf = s.lut[tid & 255];
+                    __syncthreads();
+                    if (tid < 256) s.lut[tid] = (int)(((float)cf - bmin) * delt + 0.5f);
+                    __syncthreads();
+                    for (int i = tid; i < CW * CH; i += CHECK_THREADS) s.u8[i] = (uint8_t)s.lut[s.u8[i]];
+                }
+                __syncthreads();
+            }
+            // stage 2 on UCharToFloat(offset 2) (image_proc.cpp:416-427)
+            for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
+                const int yy = i / IW, xx = i - yy * IW;
+                s.in[i] = (float)s.u8[yy * CW + xx + 2];
+            }
+            __syncthreads();
+            check_forward<Stage2>(W2, s);
+            float m1 = -10.f, m2 = -10.f;
+            int k1 = 0, k2 = 0;
+            for (int i = 0; i < RN; ++i) {
+                const float d = s.out[i];
+                m1 = (m1 < d) ? d : m1;
+                if (d > P.t2) k1++;
+            }
+            bool skip3 = (P.type_check == 0 && k1 >= mn) || (P.type_check > 0 && k1 <= 0) ||
+                         (P.type_check == 2 && k1 < mn);
+            if (!skip3) {
+                // stage 3 on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0
+                __syncthreads();
+                for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
+                    const int yy = i / IW, xx = i - yy * IW;
+                    s.in[i] = (float)s.u8[yy * CW + (P.reflection ? 46 - xx : xx)];
+                }
+                __syncthreads();
+                check_forward<Stage3>(W3, s);
+                for (int i = 0; i < RN; ++i) {
+                    const float d = s.out[i];
+                    m2 = (m2 < d) ? d : m2;
+                    if (d > P.t3) k2++;
+                }
+            }
+            bool flag = false;
+            if (P.type_check == 0) flag = (k1 >= mn || k2 >= mn);
+            else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
+            else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
+            res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
+            res.pad0 = rcols; res.pad1 = rrows;
+            __syncthreads();
+            if (flag || !P.double_check) break;
+        }
+        if (tid == 0) results[ci] = res;
+    }
+}
+
+}  // namespace ccnn
+
+// ---------------------------------------------------------------------------------
+// Launchers (host side of this translation unit)
+// ---------------------------------------------------------------------------------
+namespace ccnn {
+
+hipError_t launch_gray(const uint8_t* img, int w, int h, int ch, int step, float* gray, int gp, hipStream_t st)
+{
+    dim3 grid((w + 255) / 256, h);
+    k_gray<<<grid, 256, 0, st>>>(img, w, h, ch, step, gray, gp);
+    return hipGetLastError();
+}
+
+hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8_t* dst, int dw, int dh,
+                            hipStream_t st)
+{
+    if (dw == sw && dh == sh)
+        return hipMemcpy2DAsync(dst, dw, src, sstep, dw, dh, hipMemcpyDeviceToDevice, st);
+    const uint32_t sclx = ((uint32_t)sw << 12) / (uint32_t)dw + 1;
+    const uint32_t scly = ((uint32_t)sh << 12) / (uint32_t)dh + 1;
+    dim3 grid((dw + 255) / 256, dh);
+    k_resize_u8<<<grid, 256, 0, st>>>(src, sw, sh, sstep, dst, dw, dh, sclx, scly);
+    return hipGetLastError();
+}
+
+hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st)
+{
+    dim3 grid((L.lw + 255) / 256, L.lh);
+    k_level<<<grid, 256, 0, st>>>(gray, gp, gw, gh, L, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
+                         int ntiles, const float* W1, float* resp, hipStream_t st)
+{
+    if (ntiles <= 0) return hipSuccess;
+    k_stage1<<<ntiles, S1_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, resp);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
+                          int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st)
+{
+    if (nchunks <= 0) return hipMemsetAsync(meta, 0, sizeof(int), st);
+    k_count<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, counts);
+    k_scan<<<1, 1024, 0, st>>>(counts, nchunks, offsets, meta);
+    k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, offsets, cands);
+    return hipGetLastError();
+}
+
+hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
+                        const int* meta, const CheckParams& P, const float* W2, const float* W3,
+                        CheckResult* results, int nblocks, hipStream_t st)
+{
+    k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, results);
+    return hipGetLastError();
+}
+
+}  // namespace ccnn

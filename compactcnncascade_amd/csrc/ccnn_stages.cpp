@@ -1,0 +1,107 @@
+// Stage-level entry points of the C ABI (include/ccnn.h): each runs one kernel of the
+// detection path on host inputs and copies the result back.
+#include <cstdio>
+#include <vector>
+
+#include "ccnn.h"
+#include "ccnn_detector.h"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+    bool alloc(size_t n) { return hipMalloc(&p, n) == hipSuccess; }
+    template <class T> T* as() { return static_cast<T*>(p); }
+};
+
+bool have_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        std::printf("[ccnn] no HIP device available\n");
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccnn_stage_gray(const ccnn_image* img, float* out)
+{
+    if (!img || !img->data || !out || img->cols <= 0 || img->rows <= 0) return -1;
+    if (img->channels != 1 && img->channels != 3 && img->channels != 4) return -1;
+    if (!have_device()) return -1;
+    const size_t bytes = img->step * (size_t)(img->rows - 1) + (size_t)img->cols * img->channels;
+    DevBuf src, dst;
+    if (!src.alloc(bytes) || !dst.alloc(sizeof(float) * (size_t)img->cols * img->rows)) return -1;
+    if (hipMemcpy(src.p, img->data, bytes, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (ccnn::launch_gray(src.as<uint8_t>(), img->cols, img->rows, img->channels, (int)img->step, dst.as<float>(),
+                          img->cols, nullptr) != hipSuccess)
+        return -1;
+    if (hipMemcpy(out, dst.p, sizeof(float) * (size_t)img->cols * img->rows, hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    return 0;
+}
+
+int ccnn_stage_level(const float* gray, int gw, int gh, int lw, int lh, int mode, float* out)
+{
+    if (!gray || !out || gw < 2 || gh < 2 || lw <= 0 || lh <= 0 || mode < 0 || mode > 2) return -1;
+    if (mode == 2 && (lw != gw || lh != gh)) return -1;
+    if (!have_device()) return -1;
+    DevBuf g, o;
+    if (!g.alloc(sizeof(float) * (size_t)gw * (gh + 1)) || !o.alloc(sizeof(float) * (size_t)lw * lh)) return -1;
+    if (hipMemcpy(g.p, gray, sizeof(float) * (size_t)gw * gh, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    ccnn::LevelDesc L{};
+    L.lw = lw; L.lh = lh; L.mode = mode;
+    L.sclx = ((uint32_t)gw << 12) / (uint32_t)lw + 1;
+    L.scly = ((uint32_t)gh << 12) / (uint32_t)lh + 1;
+    if (ccnn::launch_level(g.as<float>(), gw, gw, gh, L, o.as<float>(), nullptr) != hipSuccess) return -1;
+    if (hipMemcpy(out, o.p, sizeof(float) * (size_t)lw * lh, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
+int ccnn_stage1_size(int lw, int lh, int* rw, int* rh)
+{
+    if (!rw || !rh) return -1;
+    const int c2 = ((lw - 3) >> 1) - 2, r2 = ((lh - 3) >> 1) - 2;
+    *rw = (c2 >> 1) - (ccnn::Stage1::K3W - 1);
+    *rh = (r2 >> 1) - (ccnn::Stage1::K3H - 1);
+    return (*rw > 0 && *rh > 0) ? 0 : -1;
+}
+
+int ccnn_stage1(const float* level, int lw, int lh, float* out)
+{
+    int rw, rh;
+    if (!level || !out || ccnn_stage1_size(lw, lh, &rw, &rh) < 0) return -1;
+    if (lw < 23 || lh < 27) return -1;
+    if (!have_device()) return -1;
+    ccnn::NetWeights net;
+    const unsigned char* b;
+    size_t n;
+    ccnn::embedded_model(0, &b, &n);
+    std::string err;
+    if (ccnn::load_model(b, n, 0, 1, net, err) < 0) return -1;
+    // the level image is fed through the copy mode of the fused sampler
+    ccnn::LevelDesc L{};
+    L.lw = lw; L.lh = lh; L.rw = rw; L.rh = rh; L.mode = 2; L.scale = 1.f;
+    L.tiles_x = (rw + ccnn::S1_TX - 1) / ccnn::S1_TX;
+    L.tiles_y = (rh + ccnn::S1_TY - 1) / ccnn::S1_TY;
+    DevBuf g, w, r, l;
+    if (!g.alloc(sizeof(float) * (size_t)lw * (lh + 1)) || !w.alloc(sizeof(float) * net.w.size()) ||
+        !r.alloc(sizeof(float) * (size_t)rw * rh) || !l.alloc(sizeof(L)))
+        return -1;
+    if (hipMemcpy(g.p, level, sizeof(float) * (size_t)lw * lh, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(w.p, net.w.data(), sizeof(float) * net.w.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(l.p, &L, sizeof(L), hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    if (ccnn::launch_stage1(g.as<float>(), lw, lw, lh, l.as<ccnn::LevelDesc>(), 1, L.tiles_x * L.tiles_y,
+                            w.as<float>(), r.as<float>(), nullptr) != hipSuccess)
+        return -1;
+    if (hipMemcpy(out, r.p, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
+}  // extern "C"

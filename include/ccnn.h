@@ -1,0 +1,147 @@
+/*
+ * ccnn.h -- C ABI of the MI355X compact-CNN-cascade face detector (libccnn.so).
+ *
+ * Drop-in boundary: every entry point below replaces one member of the reference's
+ * public class CompactCNNLib::FaceDetector (Bkmz21/CompactCNNCascade,
+ * prebuild/include/CompactCNNLib.h:36-194, implemented in
+ * src/CNNObjectDetector/CompactCNNLibAPI_v2.cpp). The reference exports an MSVC C++
+ * class; this library exports plain C functions over an opaque handle and POD structs,
+ * and include/CompactCNNLib.h rebuilds the reference's C++ class on top of them.
+ *
+ * All detection work runs on the GPU (HIP, gfx950). There is no CPU fallback: on a
+ * machine without a usable HIP device ccnn_init returns -1.
+ */
+#ifndef CCNN_H
+#define CCNN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FaceDetector::Pipeline / DetectMode / DetectPrecision (CompactCNNLib.h:38-59) */
+enum { CCNN_PIPELINE_CPU = 0, CCNN_PIPELINE_GPU = 1, CCNN_PIPELINE_GPU_CPU = 2 };
+enum { CCNN_DETECT_DISABLE = 0, CCNN_DETECT_SYNC = 1, CCNN_DETECT_ASYNC = 2 };
+enum { CCNN_PRECISION_DEF = 0, CCNN_PRECISION_LOW = 1, CCNN_PRECISION_NORMAL = 2,
+       CCNN_PRECISION_HIGH = 3, CCNN_PRECISION_ULTRA = 4 };
+
+/* FaceDetector::Param (CompactCNNLib.h:98-165), field for field. */
+typedef struct ccnn_param {
+    int max_image_width, max_image_height;   /* max_image_size */
+    int min_face_height;
+    int max_face_height;
+    float scale_factor;
+    int min_neighbors;
+    int detect_precision;
+    float treshold[3];
+    int drop_detect;
+    int equalize;
+    int reflection;
+    int pipeline;
+    int detect_mode;
+    int num_threads;
+    const char* models[3];                   /* accepted and ignored, as the reference ignores
+                                                them (cnn_detector_v3.cpp:917-920): the shipped
+                                                models are embedded in the library */
+    int index_output[3];
+    int device_info;
+    int cuda_device_id;                      /* HIP device ordinal */
+    int cl_platform_id;
+    int cl_device_id;
+    int facial_analysis;
+} ccnn_param;
+
+/* FaceDetector::Face (CompactCNNLib.h:73-86) */
+typedef struct ccnn_face {
+    int x, y, width, height;
+    float score;
+    int gender, smile, glasses;
+} ccnn_face;
+
+/* FaceDetector::ImageData (CompactCNNLib.h:87-96): 8-bit gray, BGR or BGRA; step in bytes. */
+typedef struct ccnn_image {
+    int cols, rows, channels;
+    unsigned char* data;
+    size_t step;
+} ccnn_image;
+
+typedef struct ccnn_detector ccnn_detector;
+
+/* FaceDetector::Param::Param() defaults (CompactCNNLib.h:148-164). */
+void ccnn_param_default(ccnn_param* p);
+
+/* FaceDetector::FaceDetector / ~FaceDetector (CompactCNNLibAPI_v2.cpp:142-180). */
+ccnn_detector* ccnn_create(void);
+void ccnn_destroy(ccnn_detector* d);
+
+/* FaceDetector::Init (CompactCNNLibAPI_v2.cpp:182-216): 0, or -1 on failure. */
+int ccnn_init(ccnn_detector* d, const ccnn_param* p);
+
+/* FaceDetector::Detect (CompactCNNLibAPI_v2.cpp:218-353). img->data is host memory.
+ * Writes min(count, capacity) faces and returns the face count; -1 if not initialised
+ * or faces == NULL. capacity < 0 means unbounded (the reference passes no capacity). */
+int ccnn_detect(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_image* img);
+
+/* As ccnn_detect, with img->data a device pointer already resident in GPU memory. */
+int ccnn_detect_device(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_image* img);
+
+/* FaceDetector::Clear (:355-372): 0, or -1 if empty. */
+int ccnn_clear(ccnn_detector* d);
+/* FaceDetector::isEmpty (:374-377) */
+int ccnn_is_empty(const ccnn_detector* d);
+/* FaceDetector::getParam / setParam (:379-404) */
+int ccnn_get_param(const ccnn_detector* d, ccnn_param* p);
+void ccnn_set_param(ccnn_detector* d, const ccnn_param* p);
+/* get/set MaxImageSize, Min/MaxObjectHeight, NumThreads (:406-460) */
+int ccnn_get_max_image_size(const ccnn_detector* d, int* width, int* height);
+void ccnn_set_max_image_size(ccnn_detector* d, int width, int height);
+int ccnn_get_min_object_height(const ccnn_detector* d);
+void ccnn_set_min_object_height(ccnn_detector* d, int h);
+int ccnn_get_max_object_height(const ccnn_detector* d);
+void ccnn_set_max_object_height(ccnn_detector* d, int h);
+int ccnn_get_num_threads(const ccnn_detector* d);
+void ccnn_set_num_threads(ccnn_detector* d, int n);
+/* FaceDetector::getGrayImage32F (:462-507): the FP32 gray image of the last frame.
+ * pipeline CCNN_PIPELINE_CPU: host pointer (library-owned copy); otherwise a device
+ * pointer. step is in elements, as the reference reports widthStep. 0 or -1. */
+int ccnn_get_gray_image(ccnn_detector* d, ccnn_image* img, int pipeline);
+
+/* ---------------------------------------------------------------------------------
+ * Stage-level entry points. They run one kernel of the detection path on host
+ * inputs and copy the result back, so that each stage can be checked on its own.
+ * All return 0 on success, -1 on bad arguments or a HIP error.
+ * --------------------------------------------------------------------------------- */
+/* Img8uToImg32fGRAY (image_proc.cpp:299-317); out: rows x cols floats. */
+int ccnn_stage_gray(const ccnn_image* img, float* out);
+/* One pyramid level (ImageResizer::FastImageResize, image_resize.cpp:564-584) of a gray
+ * image, mode 0 nearest, 1 bilinear, 2 copy; out: lh x lw. */
+int ccnn_stage_level(const float* gray, int gw, int gh, int lw, int lh, int mode, float* out);
+/* Stage-1 response map of a level image (ConvNeuralNetwork::Forward, cnn_simd_cntk.cpp:964);
+ * out: rh x rw with (rw, rh) from ccnn_stage1_size. */
+int ccnn_stage1_size(int lw, int lh, int* rw, int* rh);
+int ccnn_stage1(const float* level, int lw, int lh, float* out);
+/* Stage-2/3 check of candidate windows on a gray image (CPUCheckDetect without the drop
+ * logic): per candidate i, scale[i] and response-map position (x[i], y[i]); res[8*i..]
+ * = {flag, mod, k1, k2, m1, m2, crop_cols, crop_rows}. Settings from p. */
+int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int n,
+                     const float* scale, const int* x, const int* y, float* res);
+
+/* Record of the last ccnn_detect* call on d: the levels run (compute order) and the
+ * candidate windows with their check results. Returns the candidate count; levels[6*l..]
+ * = {scale index, lw, lh, rw, rh, mode}, n_levels receives the level count;
+ * cand[4*i..] = {level, x, y, score bits}; res[8*i..] as ccnn_stage_check. Arrays may be
+ * NULL to query the counts. */
+int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
+                    float* cand, float* res);
+
+/* Per-kernel timing of the last ccnn_detect* call (HIP events on the detector's stream),
+ * enabled by ccnn_set_profiling(d, 1). ms[0..4] = gray, stage1, compaction, check, total
+ * device time; returns 0. */
+void ccnn_set_profiling(ccnn_detector* d, int on);
+int ccnn_last_timing(const ccnn_detector* d, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCNN_H */

@@ -1,0 +1,163 @@
+"""GPU parity: every stage of the HIP path, called through the C ABI, against the
+oracle (oracle/ccnn_oracle.c) and the golden fixtures of the compiled reference.
+
+Bar: bit-exact everywhere (gray, pyramid, response maps, crops, stage-2/3 responses,
+final boxes and scores), since the HIP kernels evaluate the reference's operation order
+without contraction.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cases import CASES, GOLDEN, ORACLE_ONLY_CASES, api_param, frame_digest, params_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    from compactcnncascade_amd import api as A
+    A.load()
+    return A
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(GOLDEN, "detections.json")) as f:
+        return json.load(f)["cases"]
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def test_gray_bitexact(api, oracle):
+    rng = np.random.default_rng(1)
+    for shape in [(61, 77, 3), (33, 45, 4), (29, 31), (1080, 1920, 3)]:
+        img = rng.integers(0, 256, size=shape, dtype=np.uint8)
+        print("gray", shape, flush=True)
+        assert np.array_equal(bits(api.stage_gray(img)), bits(oracle.gray(img)))
+
+
+def test_gray_all_bgr_triples(api, oracle):
+    # every (B, G, R) triple: 4096 x 4096 image of 2^24 pixels
+    v = np.arange(1 << 24, dtype=np.uint32)
+    img = np.stack([(v & 255), (v >> 8) & 255, (v >> 16) & 255], axis=1).astype(np.uint8).reshape(4096, 4096, 3)
+    print("gray 2^24 triples", flush=True)
+    assert np.array_equal(bits(api.stage_gray(img)), bits(oracle.gray(img)))
+
+
+def test_pyramid_levels_bitexact(api, oracle):
+    from cases import photo3
+    g = oracle.gray(photo3())
+    gh, gw = g.shape
+    for (lw, lh, mode) in [(gw, gh, 2), (int(gw * 0.8), int(gh * 0.8), 0), (int(gw * 1.35), int(gh * 1.35), 0),
+                           (int(gw * 0.5), int(gh * 0.5), 1), (int(gw * 0.3), int(gh * 0.3), 1), (49, 51, 0),
+                           (47, 51, 1)]:
+        print("level", lw, lh, mode, flush=True)
+        a = api.stage_level(g, lw, lh, mode)
+        b = oracle.resize(g, lw, lh, mode) if mode != 2 else g
+        assert np.array_equal(bits(a), bits(b)), (lw, lh, mode)
+
+
+def test_stage1_bitexact(api, oracle):
+    from cases import photo3
+    rng = np.random.default_rng(5)
+    g = oracle.gray(photo3())
+    levels = [(rng.random((51, 47)) * 255).astype(np.float32),
+              (rng.random((200, 333)) * 255).astype(np.float32),
+              oracle.resize(g, 432, 324, 1),
+              g]
+    for lvl in levels:
+        print("stage1", lvl.shape, flush=True)
+        a = api.stage1(lvl)
+        b = oracle.forward(0, lvl)
+        assert a.shape == b.shape
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_stage1_shift_invariance(api):
+    # moving the level by 4 pixels moves the response map by one cell
+    rng = np.random.default_rng(9)
+    lvl = (rng.random((120, 140)) * 255).astype(np.float32)
+    a = api.stage1(lvl)
+    b = api.stage1(lvl[4:, 4:])
+    assert np.array_equal(bits(a[1:, 1:])[:b.shape[0], :b.shape[1]], bits(b)[:a.shape[0] - 1, :a.shape[1] - 1])
+
+
+def test_check_stage_bitexact(api, oracle):
+    from cases import vga
+    f = vga(3)
+    over = dict(min_face=40)
+    p = params_for(f, over)
+    _, tr = oracle.detect(f, p, trace_cap=100000)
+    scales = tr["scales"]
+    g = oracle.gray(f)
+    sc = scales[tr["c_scl"]]
+    res = api.stage_check(api_param(f, over), g, sc, tr["c_x"], tr["c_y"])
+    n = 0
+    for i in range(len(sc)):
+        r = oracle.check_candidate(p, g, sc[i], tr["c_x"][i] * 4, tr["c_y"][i] * 4)
+        assert int(res[i, 0]) == int(r[0]) and int(res[i, 1]) == int(r[1]), i
+        assert int(res[i, 2]) == int(r[2]) and int(res[i, 3]) == int(r[3]), i
+        assert bits(res[i, 4:6]).tolist() == bits(r[4:6]).tolist(), i
+        n += 1
+    assert n > 10
+
+
+@pytest.mark.parametrize("name", list(CASES) + list(ORACLE_ONLY_CASES))
+def test_detect_matches_reference(api, golden, name):
+    build, over = {**CASES, **ORACLE_ONLY_CASES}[name]
+    f = build()
+    g = golden[name]
+    assert frame_digest(f) == g["digest"], "input frame differs from the fixture's"
+    print("detect", name, f.shape, flush=True)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    n, faces = d.Detect(f)
+    exp = g["faces"]
+    assert n == len(exp)
+    got = [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces]
+    assert got == [e[:4] for e in exp]
+    assert [int(v) for v in bits(faces["score"])] == g["score_bits"]
+
+
+def test_trace_matches_oracle(api, oracle):
+    from cases import photo3
+    f = photo3()
+    over = dict(min_face=20)
+    p = params_for(f, over)
+    _, tr = oracle.detect(f, p, trace_cap=200000)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    d.Detect(f)
+    t = d.last_trace()
+    # levels in compute order
+    assert t["levels"][:, 0].tolist() == tr["lvl_scl"].tolist()
+    assert t["levels"][:, 1:5].tolist() == np.stack([tr["lvl_w"], tr["lvl_h"], tr["lvl_rw"], tr["lvl_rh"]], 1).tolist()
+    # candidate windows: same list, same order, same stage-1 scores
+    cand = t["cand"]
+    assert len(cand) == len(tr["c_x"])
+    assert cand[:, 1].astype(int).tolist() == tr["c_x"].tolist()
+    assert cand[:, 2].astype(int).tolist() == tr["c_y"].tolist()
+    assert bits(cand[:, 3]).tolist() == bits(tr["c_score"]).tolist()
+    # check outcome of every candidate the reference evaluated
+    ev = np.isin(tr["c_status"], [2, 3])
+    res = t["res"][ev]
+    assert (res[:, 0].astype(int) == (tr["c_status"][ev] == 2)).all()
+    assert res[:, 1].astype(int).tolist() == tr["c_mod"][ev].tolist()
+    assert res[:, 2].astype(int).tolist() == tr["c_k1"][ev].tolist()
+    assert bits(res[:, 4]).tolist() == bits(tr["c_m1"][ev]).tolist()
+
+
+def test_repeated_detect_is_stable(api):
+    from cases import vga
+    f = vga(0)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, dict(min_face=40))) == 0
+    first = d.Detect(f)[1]
+    for _ in range(3):
+        again = d.Detect(f)[1]
+        assert np.array_equal(first, again)
