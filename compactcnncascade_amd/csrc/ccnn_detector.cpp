@@ -320,15 +320,15 @@ Detector::Detector()
 Detector::~Detector()
 {
     release();
-    if (stream_) hipStreamDestroy(stream_);
+    if (stream_) (void)hipStreamDestroy(stream_);
     for (auto& e : ev_)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
 }
 
 void Detector::release()
 {
-    auto F = [](auto*& p) { if (p) { hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { hipHostFree((void*)p); p = nullptr; } };
+    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
+    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
     F(d_img_); F(d_pre_); F(d_gray_); F(d_resp_); F(d_levels_); F(d_counts_); F(d_offsets_); F(d_meta_);
     F(d_cands_); F(d_results_);
     for (auto& w : d_w_) F(w);
@@ -425,8 +425,8 @@ int Detector::ensure_buffers(int w, int h)
     if (w <= cap_w_ && h <= cap_h_ && d_gray_) return 0;
     // device buffers only grow
     const int W = std::max(w, cap_w_), H = std::max(h, cap_h_);
-    auto F = [](auto*& p) { if (p) { hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { hipHostFree((void*)p); p = nullptr; } };
+    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
+    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
     F(d_img_); F(d_pre_); F(d_gray_); F(d_resp_); F(d_levels_); F(d_counts_); F(d_offsets_); F(d_meta_);
     F(d_cands_); F(d_results_);
     FH(h_meta_); FH(h_cands_); FH(h_results_);
@@ -487,8 +487,8 @@ int Detector::plan_frame(int gw, int gh)
     ntiles_ = tiles;
     nchunks_ = chunks;
     resp_total_ = roff;
-    auto F = [](auto*& p) { if (p) { hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { hipHostFree((void*)p); p = nullptr; } };
+    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
+    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
     if (roff > resp_cap_) {
         F(d_resp_); F(d_cands_); F(d_results_);
         HIP_OK(hipMalloc(&d_resp_, sizeof(float) * std::max<size_t>(roff, 1)));
@@ -554,7 +554,7 @@ int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_fa
     } else {
         const size_t bytes = img.step * (size_t)(h - 1) + (size_t)w * ch;
         if (bytes > img_bytes_) {
-            if (d_img_) hipFree(d_img_);
+            if (d_img_) (void)hipFree(d_img_);
             d_img_ = nullptr;
             HIP_OK(hipMalloc(&d_img_, bytes));
             img_bytes_ = bytes;
@@ -628,8 +628,8 @@ int Detector::run_frame(const uint8_t* src, int w, int h, int ch, size_t step, s
         }
     }
     if (profiling_) {
-        for (int i = 0; i < 4; ++i) hipEventElapsedTime(&last_ms_[i == 0 ? 0 : i], ev_[i], ev_[i + 1]);
-        hipEventElapsedTime(&last_ms_[4], ev_[0], ev_[4]);
+        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&last_ms_[i == 0 ? 0 : i], ev_[i], ev_[i + 1]);
+        (void)hipEventElapsedTime(&last_ms_[4], ev_[0], ev_[4]);
     }
     last_count_ = count;
     last_levels_ = levels_;
@@ -774,7 +774,7 @@ int Detector::last_trace(int* n_levels, int* levels, int cap, float* cand, float
         const CheckResult& r = last_results_[i];
         float* o = res + 8 * i;
         o[0] = (float)r.flag; o[1] = (float)r.mod; o[2] = (float)r.k1; o[3] = (float)r.k2;
-        o[4] = r.m1; o[5] = r.m2; o[6] = (float)r.pad0; o[7] = (float)r.pad1;
+        o[4] = r.m1; o[5] = r.m2; o[6] = (float)r.n2; o[7] = (float)r.n3;
     }
     return last_count_;
 }
@@ -817,12 +817,12 @@ int Detector::stage_check(const float* gray, int gw, int gh, int n, const float*
         hipStreamSynchronize(stream_) != hipSuccess ||
         hipMemcpy(out.data(), d_r, sizeof(CheckResult) * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = -1;
-    hipFree(d_g); hipFree(d_l); hipFree(d_c); hipFree(d_r); hipFree(d_m);
+    (void)hipFree(d_g); (void)hipFree(d_l); (void)hipFree(d_c); (void)hipFree(d_r); (void)hipFree(d_m);
     if (rc < 0) return -1;
     for (int i = 0; i < n; ++i) {
         float* o = res + 8 * i;
         o[0] = (float)out[i].flag; o[1] = (float)out[i].mod; o[2] = (float)out[i].k1; o[3] = (float)out[i].k2;
-        o[4] = out[i].m1; o[5] = out[i].m2; o[6] = (float)out[i].pad0; o[7] = (float)out[i].pad1;
+        o[4] = out[i].m1; o[5] = out[i].m2; o[6] = (float)out[i].n2; o[7] = (float)out[i].n3;
     }
     return 0;
 }

@@ -78,7 +78,7 @@ struct CheckResult {
     int mod;              // crop variant of the final evaluation
     int k1, k2;           // responses above the stage-2/3 thresholds
     float m1, m2;         // maxima of the stage-2/3 responses
-    int pad0, pad1;
+    int n2, n3;           // stage-2 and stage-3 network evaluations performed
 };
 
 // Scalar parameters of the check kernel.

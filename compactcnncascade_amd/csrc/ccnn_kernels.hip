@@ -586,6 +586,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
         const float inv = 1.f / scale;
         const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
         CheckResult res;
+        int n2 = 0, n3 = 0;
         for (int mod = 0; mod < 2; ++mod) {
             // crop geometry (cnn_detector_v3.cpp:1491-1556)
             int x = (int)((float)(pxp - 6) * inv);
@@ -648,6 +649,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
             }
             __syncthreads();
             check_forward<Stage2>(W2, s);
+            n2++;
             float m1 = -10.f, m2 = -10.f;
             int k1 = 0, k2 = 0;
             for (int i = 0; i < RN; ++i) {
@@ -666,6 +668,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
                 }
                 __syncthreads();
                 check_forward<Stage3>(W3, s);
+                n3++;
                 for (int i = 0; i < RN; ++i) {
                     const float d = s.out[i];
                     m2 = (m2 < d) ? d : m2;
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
             else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
             else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
             res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
-            res.pad0 = rcols; res.pad1 = rrows;
+            res.n2 = n2; res.n3 = n3;
             __syncthreads();
             if (flag || !P.double_check) break;
         }

@@ -10,7 +10,7 @@ namespace {
 
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
     bool alloc(size_t n) { return hipMalloc(&p, n) == hipSuccess; }
     template <class T> T* as() { return static_cast<T*>(p); }
 };

@@ -123,14 +123,16 @@ int ccnn_stage1_size(int lw, int lh, int* rw, int* rh);
 int ccnn_stage1(const float* level, int lw, int lh, float* out);
 /* Stage-2/3 check of candidate windows on a gray image (CPUCheckDetect without the drop
  * logic): per candidate i, scale[i] and response-map position (x[i], y[i]); res[8*i..]
- * = {flag, mod, k1, k2, m1, m2, crop_cols, crop_rows}. Settings from p. */
+ * = {flag, mod, k1, k2, m1, m2, n2, n3}: acceptance, crop variant of the final evaluation,
+ * stage-2/3 counts and maxima of that evaluation, and the number of stage-2 and stage-3
+ * network evaluations performed. Settings from p. */
 int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int n,
                      const float* scale, const int* x, const int* y, float* res);
 
 /* Record of the last ccnn_detect* call on d: the levels run (compute order) and the
  * candidate windows with their check results. Returns the candidate count; levels[6*l..]
  * = {scale index, lw, lh, rw, rh, mode}, n_levels receives the level count;
- * cand[4*i..] = {level, x, y, score bits}; res[8*i..] as ccnn_stage_check. Arrays may be
+ * cand[4*i..] = {level, x, y, score}; res[8*i..] as ccnn_stage_check. Arrays may be
  * NULL to query the counts. */
 int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
                     float* cand, float* res);
