@@ -62,6 +62,7 @@ def load():
     L.ccnn_init.argtypes = [_P, ctypes.POINTER(CParam)]
     L.ccnn_detect.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
     L.ccnn_detect_device.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
+    L.ccnn_detect_batch.argtypes = [_P, _I, _P, _I, _P, _I, _P]
     L.ccnn_clear.argtypes = [_P]
     L.ccnn_is_empty.argtypes = [_P]
     L.ccnn_get_param.argtypes = [_P, ctypes.POINTER(CParam)]
@@ -82,6 +83,7 @@ def load():
     L.ccnn_stage_check.argtypes = [ctypes.POINTER(CParam), _P, _I, _I, _I, _P, _P, _P, _P]
     L.ccnn_last_trace.argtypes = [_P, ctypes.POINTER(_I), _P, _I, _P, _P]
     L.ccnn_set_profiling.argtypes = [_P, _I]
+    L.ccnn_last_error.argtypes = [_I]
     L.ccnn_last_timing.argtypes = [_P, _P]
     _lib = L
     return L
@@ -213,6 +215,24 @@ class FaceDetector:
         n = self._L.ccnn_detect_device(self._h, _ptr(faces), 4096, ctypes.byref(ci))
         return n, faces[:max(0, min(n, 4096))]
 
+    def DetectBatch(self, images=None, device_frames=None, capacity=256):
+        """Pipelined detection of several frames. images: list of host arrays, or
+        device_frames: list of (device pointer, cols, rows, channels, step).
+        Returns a list of (count, faces) per frame."""
+        if images is not None:
+            keep = [_image(im) for im in images]
+            cimgs = (CImage * len(keep))(*[ci for _, ci in keep])
+            device = 0
+        else:
+            cimgs = (CImage * len(device_frames))(*[CImage(c, r, ch, p, st) for (p, c, r, ch, st) in device_frames])
+            device = 1
+        n = len(cimgs)
+        faces = np.zeros(max(1, n * capacity), FACE_DTYPE)
+        counts = np.zeros(max(1, n), np.int32)
+        if self._L.ccnn_detect_batch(self._h, n, cimgs, device, _ptr(faces), capacity, _ptr(counts)) != 0:
+            raise RuntimeError("ccnn_detect_batch failed")
+        return [(int(counts[i]), faces[i * capacity:i * capacity + min(int(counts[i]), capacity)]) for i in range(n)]
+
     def Clear(self):
         return self._L.ccnn_clear(self._h)
 
@@ -269,10 +289,13 @@ class FaceDetector:
         self._L.ccnn_set_profiling(self._h, int(on))
 
     def last_timing(self):
-        ms = np.zeros(5, np.float32)
+        """Per-kernel milliseconds of the last frame, and totals since set_profiling."""
+        ms = np.zeros(11, np.float32)
         self._L.ccnn_last_timing(self._h, _ptr(ms))
-        return dict(gray=float(ms[0]), stage1=float(ms[1]), compact=float(ms[2]), check=float(ms[3]),
-                    total=float(ms[4]))
+        keys = ("gray", "stage1", "compact", "check", "total")
+        last = {k: float(ms[i]) for i, k in enumerate(keys)}
+        totals = {k: float(ms[5 + i]) for i, k in enumerate(keys)}
+        return dict(last=last, totals=totals, frames=int(ms[10]))
 
     def last_trace(self):
         """Levels run and candidate windows of the last Detect, with their check results."""
@@ -283,6 +306,11 @@ class FaceDetector:
         res = np.zeros((max(1, n), 8), np.float32)
         self._L.ccnn_last_trace(self._h, ctypes.byref(nl), _ptr(levels), n, _ptr(cand), _ptr(res))
         return dict(levels=levels[:nl.value], cand=cand[:n], res=res[:n])
+
+
+def last_error(reset=True):
+    """Last HIP error code met by the library on this thread (0 if none)."""
+    return load().ccnn_last_error(int(reset))
 
 
 # -- stage-level entry points --------------------------------------------------------

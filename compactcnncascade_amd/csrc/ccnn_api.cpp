@@ -1,5 +1,6 @@
 // C ABI (include/ccnn.h) and the C++ FaceDetector facade (include/CompactCNNLib.h).
 // Reference: Bkmz21/CompactCNNCascade src/CNNObjectDetector/CompactCNNLibAPI_v2.cpp.
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <mutex>
@@ -92,6 +93,24 @@ int ccnn_detect(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_ima
 int ccnn_detect_device(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_image* img)
 {
     return detect_common(d, faces, capacity, img, true);
+}
+
+int ccnn_detect_batch(ccnn_detector* d, int n, const ccnn_image* imgs, int device, ccnn_face* faces, int capacity,
+                      int* counts)
+{
+    if (!d || d->det.empty()) {
+        std::printf("[CompactCNNLibAPI] FaceDetector no initialized!\n");
+        return -1;
+    }
+    if (n < 0 || (n > 0 && (!imgs || !counts || (capacity > 0 && !faces)))) return -1;
+    std::vector<std::vector<ccnn_face>> out;
+    if (d->det.detect_batch(n, imgs, device != 0, out) < 0) return -1;
+    for (int i = 0; i < n; ++i) {
+        counts[i] = (int)out[i].size();
+        const int m = std::min(counts[i], std::max(capacity, 0));
+        for (int k = 0; k < m; ++k) faces[(size_t)i * capacity + k] = out[i][k];
+    }
+    return 0;
 }
 
 int ccnn_clear(ccnn_detector* d)
@@ -205,6 +224,11 @@ int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
 {
     if (!d) return -1;
     return d->det.last_trace(n_levels, levels, cap, cand, res);
+}
+
+int ccnn_last_error(int reset)
+{
+    return ccnn::last_error(reset != 0);
 }
 
 void ccnn_set_profiling(ccnn_detector* d, int on)

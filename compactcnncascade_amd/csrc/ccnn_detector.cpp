@@ -9,15 +9,26 @@
 
 namespace ccnn {
 
+// last HIP error seen by this library on the calling thread (0 if none since the reset)
+thread_local int g_last_error = 0;
+
 #define HIP_OK(expr)                                                                         \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \
         if (e_ != hipSuccess) {                                                              \
+            g_last_error = (int)e_;                                                          \
             std::fprintf(stderr, "[ccnn] HIP error %s at %s:%d\n", hipGetErrorString(e_),     \
                          __FILE__, __LINE__);                                                \
             return -1;                                                                       \
         }                                                                                    \
     } while (0)
+
+int last_error(bool reset)
+{
+    const int e = g_last_error;
+    if (reset) g_last_error = 0;
+    return e;
+}
 
 // ---------------------------------------------------------------------------------
 // Model loading: binary format v1.0-1.1 (cnn_simd_cntk.cpp:42-404)
@@ -309,9 +320,42 @@ std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s)
     return dets;
 }
 
+
 // ---------------------------------------------------------------------------------
 // Detector
 // ---------------------------------------------------------------------------------
+namespace {
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) { (void)hipFree((void*)p); p = nullptr; }
+}
+template <class T>
+void hfree(T*& p)
+{
+    if (p) { (void)hipHostFree((void*)p); p = nullptr; }
+}
+template <class T>
+int dalloc(T*& p, size_t n)
+{
+    dfree(p);
+    return hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
+}
+template <class T>
+int halloc(T*& p, size_t n)
+{
+    hfree(p);
+    return hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
+}
+
+// candidates read back with the first copy; more need a second round trip
+constexpr size_t PREFETCH = 4096;
+// pinned read-back buffers are capped; larger candidate lists are copied to pageable memory
+constexpr size_t PINNED_CAP = 1u << 20;
+
+}  // namespace
+
 Detector::Detector()
 {
     ccnn_param_default(&param);
@@ -319,23 +363,21 @@ Detector::Detector()
 
 Detector::~Detector()
 {
-    release();
-    if (stream_) (void)hipStreamDestroy(stream_);
-    for (auto& e : ev_)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& s : slots_) release_slot(s);
+    for (auto& w : d_w_) dfree(w);
+    for (Plan* p : plans_) delete p;
 }
 
-void Detector::release()
+void Detector::release_slot(Slot& s)
 {
-    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
-    F(d_img_); F(d_pre_); F(d_gray_); F(d_resp_); F(d_levels_); F(d_counts_); F(d_offsets_); F(d_meta_);
-    F(d_cands_); F(d_results_);
-    for (auto& w : d_w_) F(w);
-    FH(h_meta_); FH(h_cands_); FH(h_results_);
-    cap_w_ = cap_h_ = 0;
-    img_bytes_ = resp_cap_ = chunk_cap_ = cand_cap_ = h_cap_ = 0;
-    plan_gw_ = plan_gh_ = -1;
+    if (s.st) (void)hipStreamSynchronize(s.st);
+    dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_levels); dfree(s.d_counts);
+    dfree(s.d_offsets); dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
+    hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
+    for (auto& e : s.ev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (s.st) { (void)hipStreamDestroy(s.st); s.st = nullptr; }
+    s = Slot();
 }
 
 int Detector::init_device()
@@ -347,9 +389,11 @@ int Detector::init_device()
     }
     device_ = (param.cuda_device_id >= 0 && param.cuda_device_id < count) ? param.cuda_device_id : 0;
     HIP_OK(hipSetDevice(device_));
-    if (!stream_) HIP_OK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    for (auto& e : ev_)
-        if (!e) HIP_OK(hipEventCreate(&e));
+    for (auto& s : slots_) {
+        if (!s.st) HIP_OK(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+        for (auto& e : s.ev)
+            if (!e) HIP_OK(hipEventCreate(&e));
+    }
     if (param.device_info) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device_) == hipSuccess)
@@ -372,19 +416,20 @@ int Detector::build()
     built_ = false;
     empty_ = true;
     if (init_device() < 0) return -1;
+    for (auto& s : slots_)
+        if (s.busy) { (void)hipStreamSynchronize(s.st); s.busy = false; }
     set_ = effective_settings(param);
-    for (int s = 0; s < 3; ++s) {
+    for (int st = 0; st < 3; ++st) {
         const unsigned char* b;
         size_t n;
-        embedded_model(s, &b, &n);
+        embedded_model(st, &b, &n);
         std::string err;
-        if (load_model(b, n, s, param.index_output[s], net_[s], err) < 0) {
+        if (load_model(b, n, st, param.index_output[st], net_[st], err) < 0) {
             std::printf("[ccnn] %s\n", err.c_str());
             return -1;
         }
-        if (!d_w_[s]) HIP_OK(hipMalloc(&d_w_[s], sizeof(float) * 4096));
-        if (net_[s].w.size() > 4096) return -1;
-        HIP_OK(hipMemcpy(d_w_[s], net_[s].w.data(), sizeof(float) * net_[s].w.size(), hipMemcpyHostToDevice));
+        if (dalloc(d_w_[st], net_[st].w.size()) < 0) return -1;
+        HIP_OK(hipMemcpy(d_w_[st], net_[st].w.data(), sizeof(float) * net_[st].w.size(), hipMemcpyHostToDevice));
     }
     float sf;
     scales_ = init_scales(param, net_[0].min_h, gray_only_, &pre_scale_, &sf);
@@ -405,7 +450,9 @@ int Detector::build()
         ext_w_ = net_[0].min_w;
         ext_h_ = net_[0].min_h;
     }
-    if (ensure_buffers(param.max_image_width, param.max_image_height) < 0) return -1;
+    for (Plan* pl : plans_) delete pl;
+    plans_.clear();
+    for (auto& s : slots_) s.levels_plan = nullptr;
     built_ = true;
     empty_ = false;
     return 0;
@@ -414,47 +461,23 @@ int Detector::build()
 int Detector::clear()
 {
     if (empty_) return -1;
+    for (auto& s : slots_)
+        if (s.busy) { (void)hipStreamSynchronize(s.st); s.busy = false; }
     empty_ = true;
     built_ = false;
-    plan_gw_ = plan_gh_ = -1;
-    return 0;
-}
-
-int Detector::ensure_buffers(int w, int h)
-{
-    if (w <= cap_w_ && h <= cap_h_ && d_gray_) return 0;
-    // device buffers only grow
-    const int W = std::max(w, cap_w_), H = std::max(h, cap_h_);
-    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
-    F(d_img_); F(d_pre_); F(d_gray_); F(d_resp_); F(d_levels_); F(d_counts_); F(d_offsets_); F(d_meta_);
-    F(d_cands_); F(d_results_);
-    FH(h_meta_); FH(h_cands_); FH(h_results_);
-    img_bytes_ = (size_t)W * H * 4;
-    HIP_OK(hipMalloc(&d_img_, img_bytes_));
-    HIP_OK(hipMalloc(&d_pre_, (size_t)W * H));
-    gpitch_ = (W + 63) / 64 * 64;
-    HIP_OK(hipMalloc(&d_gray_, sizeof(float) * (size_t)gpitch_ * (H + 1)));
-    // response maps of all levels: bounded by the level pixel count / 16 (stride 4) with
-    // the largest scale (3x) -> allow 9 * W * H / 16 * levels; compute exactly per plan
-    // and grow lazily in plan_frame.
-    resp_cap_ = 0;
-    HIP_OK(hipMalloc(&d_levels_, sizeof(LevelDesc) * MAX_LEVELS));
-    HIP_OK(hipMalloc(&d_meta_, sizeof(int) * 16));
-    HIP_OK(hipHostMalloc(&h_meta_, sizeof(int) * 16));
-    cap_w_ = W;
-    cap_h_ = H;
-    plan_gw_ = plan_gh_ = -1;
     return 0;
 }
 
 // Level table for a gray image of gw x gh: levels scl_max -> 0 (RunCPUDetect order,
 // which is also the RunCheckDetectAsync order), skip rule (:2083-2088), resize mode
 // (:2093-2104), response size (getOutputImgSize, cnn_simd_cntk.cpp:997-1012).
-int Detector::plan_frame(int gw, int gh)
+const Plan* Detector::plan_for(int gw, int gh)
 {
-    if (gw == plan_gw_ && gh == plan_gh_) return 0;
-    levels_.clear();
+    for (const Plan* p : plans_)
+        if (p->gw == gw && p->gh == gh) return p;
+    Plan* P = new Plan();
+    P->gw = gw;
+    P->gh = gh;
     int tiles = 0, chunks = 0;
     size_t roff = 0;
     for (int scl = (int)scales_.size() - 1; scl >= 0; --scl) {
@@ -464,11 +487,10 @@ int Detector::plan_frame(int gw, int gh)
         size_mul(gw, gh, L.scale, &L.lw, &L.lh);
         if (L.lw < ext_w_ || L.lh < ext_h_) continue;
         L.mode = (L.scale != 1.f) ? (L.scale > 0.7f ? 0 : 1) : 2;
-        if (L.mode != 2 && L.lw == gw && L.lh == gh) L.mode = 2;   // FastImageResize copies equal sizes
+        if (L.lw == gw && L.lh == gh) L.mode = 2;       // FastImageResize copies equal sizes
         L.sclx = ((uint32_t)gw << 12) / (uint32_t)L.lw + 1;
         L.scly = ((uint32_t)gh << 12) / (uint32_t)L.lh + 1;
-        const int c1 = L.lw - 3, r1 = L.lh - 3;
-        const int c2 = (c1 >> 1) - 2, r2 = (r1 >> 1) - 2;
+        const int c2 = ((L.lw - 3) >> 1) - 2, r2 = ((L.lh - 3) >> 1) - 2;
         L.rw = (c2 >> 1) - (Stage1::K3W - 1);
         L.rh = (r2 >> 1) - (Stage1::K3H - 1);
         if (L.rw <= 0 || L.rh <= 0) { L.rw = 0; L.rh = 0; }
@@ -481,52 +503,77 @@ int Detector::plan_frame(int gw, int gh)
         L.chunk_off = chunks;
         L.chunks = (L.rw * L.rh + CMP_CHUNK - 1) / CMP_CHUNK;
         chunks += L.chunks;
-        levels_.push_back(L);
+        P->levels.push_back(L);
     }
-    if (levels_.size() > (size_t)MAX_LEVELS) return -1;
-    ntiles_ = tiles;
-    nchunks_ = chunks;
-    resp_total_ = roff;
-    auto F = [](auto*& p) { if (p) { (void)hipFree((void*)p); p = nullptr; } };
-    auto FH = [](auto*& p) { if (p) { (void)hipHostFree((void*)p); p = nullptr; } };
-    if (roff > resp_cap_) {
-        F(d_resp_); F(d_cands_); F(d_results_);
-        HIP_OK(hipMalloc(&d_resp_, sizeof(float) * std::max<size_t>(roff, 1)));
-        HIP_OK(hipMalloc(&d_cands_, sizeof(Candidate) * std::max<size_t>(roff, 1)));
-        HIP_OK(hipMalloc(&d_results_, sizeof(CheckResult) * std::max<size_t>(roff, 1)));
-        resp_cap_ = roff;
-        cand_cap_ = roff;
+    P->ntiles = tiles;
+    P->nchunks = chunks;
+    P->resp_total = roff;
+    if (plans_.size() >= 8) {          // a handful of frame sizes stay cached
+        for (size_t k = 0; k < plans_.size(); ++k) {
+            bool used = false;
+            for (auto& s : slots_) used |= s.busy && s.plan == plans_[k];
+            if (used) continue;
+            for (auto& s : slots_)
+                if (s.levels_plan == plans_[k]) s.levels_plan = nullptr;
+            delete plans_[k];
+            plans_.erase(plans_.begin() + k);
+            break;
+        }
     }
-    if ((size_t)chunks > chunk_cap_) {
-        F(d_counts_); F(d_offsets_);
-        HIP_OK(hipMalloc(&d_counts_, sizeof(int) * std::max(chunks, 1)));
-        HIP_OK(hipMalloc(&d_offsets_, sizeof(int) * std::max(chunks, 1)));
-        chunk_cap_ = chunks;
+    plans_.push_back(P);
+    return P;
+}
+
+int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& plan)
+{
+    if (img_bytes > s.img_bytes) {
+        if (dalloc(s.d_img, img_bytes) < 0) return -1;
+        s.img_bytes = img_bytes;
     }
-    const size_t hc = std::min<size_t>(std::max<size_t>(roff, 1), 1u << 20);
-    if (hc > h_cap_) {
-        FH(h_cands_); FH(h_results_);
-        HIP_OK(hipHostMalloc(&h_cands_, sizeof(Candidate) * hc));
-        HIP_OK(hipHostMalloc(&h_results_, sizeof(CheckResult) * hc));
-        h_cap_ = hc;
+    const size_t pre = (size_t)w * h;
+    if (pre > s.pre_bytes) {
+        if (dalloc(s.d_pre, pre) < 0) return -1;
+        s.pre_bytes = pre;
     }
-    if (!levels_.empty())
-        HIP_OK(hipMemcpy(d_levels_, levels_.data(), sizeof(LevelDesc) * levels_.size(), hipMemcpyHostToDevice));
-    plan_gw_ = gw;
-    plan_gh_ = gh;
+    const int gp = (w + 63) / 64 * 64;
+    const size_t ge = (size_t)gp * (h + 1);
+    if (ge > s.gray_elems || gp > s.gpitch) {
+        if (dalloc(s.d_gray, std::max(ge, s.gray_elems)) < 0) return -1;
+        s.gray_elems = std::max(ge, s.gray_elems);
+    }
+    s.gpitch = gp;
+    if (!s.d_levels && dalloc(s.d_levels, MAX_LEVELS) < 0) return -1;
+    if (!s.d_meta && dalloc(s.d_meta, 16) < 0) return -1;
+    if (!s.h_meta && halloc(s.h_meta, 16) < 0) return -1;
+    if (plan.resp_total > s.resp_cap) {
+        const size_t n = plan.resp_total;
+        if (dalloc(s.d_resp, n) < 0 || dalloc(s.d_cands, n) < 0 || dalloc(s.d_results, n) < 0 ||
+            dalloc(s.d_s2, 2 * n) < 0 || dalloc(s.d_s3, 2 * n) < 0)
+            return -1;
+        s.resp_cap = s.cand_cap = n;
+    }
+    if ((size_t)plan.nchunks > s.chunk_cap) {
+        if (dalloc(s.d_counts, plan.nchunks) < 0 || dalloc(s.d_offsets, plan.nchunks) < 0) return -1;
+        s.chunk_cap = plan.nchunks;
+    }
+    const size_t hc = std::min(std::max<size_t>(plan.resp_total, PREFETCH), PINNED_CAP);
+    if (hc > s.h_cap) {
+        if (halloc(s.h_cands, hc) < 0 || halloc(s.h_results, hc) < 0) return -1;
+        s.h_cap = hc;
+    }
     return 0;
 }
 
-int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces)
+// Size guards and re-initialisation at the start of CNNDetector::Detect
+// (cnn_detector_v3.cpp:2483-2547). Returns 1 to run the frame, 0 for "no faces", -1 on error.
+int Detector::prepare(const ccnn_image& img)
 {
-    faces.clear();
     if (empty_ || !built_) {
         if (build() < 0) {
             std::printf("[CNNDetector] CNN detector no initialized!\n");
             return -1;
         }
     }
-    HIP_OK(hipSetDevice(device_));
     const int w = img.cols, h = img.rows, ch = img.channels;
     if (w < ext_w_ || h < ext_h_) return 0;                                  // :2494-2497
     if (w > param.max_image_width || h > param.max_image_height) {           // :2499-2513
@@ -548,91 +595,118 @@ int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_fa
         return -1;
     }
     if (!img.data) return -1;
-    const uint8_t* src;
-    if (device_ptr) {
-        src = img.data;
-    } else {
-        const size_t bytes = img.step * (size_t)(h - 1) + (size_t)w * ch;
-        if (bytes > img_bytes_) {
-            if (d_img_) (void)hipFree(d_img_);
-            d_img_ = nullptr;
-            HIP_OK(hipMalloc(&d_img_, bytes));
-            img_bytes_ = bytes;
-        }
-        HIP_OK(hipMemcpyAsync(d_img_, img.data, bytes, hipMemcpyHostToDevice, stream_));
-        src = d_img_;
-    }
-    return run_frame(src, w, h, ch, img.step, faces);
+    return 1;
 }
 
-int Detector::run_frame(const uint8_t* src, int w, int h, int ch, size_t step, std::vector<ccnn_face>& faces)
+CheckParams Detector::check_params() const
 {
-    hipStream_t st = stream_;
+    CheckParams P;
+    P.t2 = set_.t2; P.t3 = set_.t3; P.type_check = set_.type_check; P.min_neighbors = set_.min_neighbors;
+    P.adapt = set_.adapt; P.double_check = set_.double_check; P.equalize = set_.equalize;
+    P.reflection = set_.reflection;
+    return P;
+}
+
+// Launches one frame on slot s: upload (host frames), optional 8-bit pre-resize, gray,
+// stage 1 over all levels, compaction, stage-2/3 checks, asynchronous read-back.
+int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
+{
+    HIP_OK(hipSetDevice(device_));
+    const int w = img.cols, h = img.rows, ch = img.channels;
     int gw = w, gh = h, gch = ch;
-    size_t gstep = step;
-    if (profiling_) HIP_OK(hipEventRecord(ev_[0], st));
-    if (gray_only_ && pre_scale_ > 0.f && ch == 1) {
+    size_t gstep = img.step;
+    const bool pre = gray_only_ && pre_scale_ > 0.f && ch == 1;
+    if (pre) size_mul(w, h, pre_scale_, &gw, &gh);
+    const Plan* plan = plan_for(gw, gh);
+    const size_t bytes = img.step * (size_t)(h - 1) + (size_t)w * ch;
+    if (ensure_slot(s, std::max(w, gw), std::max(h, gh), device_ptr ? 0 : bytes, *plan) < 0) return -1;
+    hipStream_t st = s.st;
+    const uint8_t* src = img.data;
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[0], st));
+    if (!device_ptr) {
+        HIP_OK(hipMemcpyAsync(s.d_img, img.data, bytes, hipMemcpyHostToDevice, st));
+        src = s.d_img;
+    }
+    if (pre) {
         // 8-bit bilinear pre-resize of a gray frame (cnn_detector_v3.cpp:2531-2539)
-        size_mul(w, h, pre_scale_, &gw, &gh);
-        HIP_OK(launch_resize_u8(src, w, h, (int)step, d_pre_, gw, gh, st));
-        src = d_pre_;
+        HIP_OK(launch_resize_u8(src, w, h, (int)img.step, s.d_pre, gw, gh, st));
+        src = s.d_pre;
         gstep = gw;
         gch = 1;
     }
-    HIP_OK(launch_gray(src, gw, gh, gch, (int)gstep, d_gray_, gpitch_, st));
-    last_gw_ = gw;
-    last_gh_ = gh;
-    if (plan_frame(gw, gh) < 0) return -1;
-    const int nl = (int)levels_.size();
-    if (profiling_) HIP_OK(hipEventRecord(ev_[1], st));
-    HIP_OK(launch_stage1(d_gray_, gpitch_, gw, gh, d_levels_, nl, ntiles_, d_w_[0], d_resp_, st));
-    if (profiling_) HIP_OK(hipEventRecord(ev_[2], st));
-    HIP_OK(launch_compact(d_resp_, d_levels_, nl, nchunks_, set_.t1, d_counts_, d_offsets_, d_meta_, d_cands_, st));
-    if (profiling_) HIP_OK(hipEventRecord(ev_[3], st));
-    const bool checks = set_.detect_mode != CCNN_DETECT_DISABLE;
-    if (checks && nl > 0) {
-        CheckParams P;
-        P.t2 = set_.t2; P.t3 = set_.t3; P.type_check = set_.type_check; P.min_neighbors = set_.min_neighbors;
-        P.adapt = set_.adapt; P.double_check = set_.double_check; P.equalize = set_.equalize;
-        P.reflection = set_.reflection;
-        HIP_OK(launch_check(d_gray_, gpitch_, gw, gh, d_levels_, d_cands_, d_meta_, P, d_w_[1], d_w_[2], d_results_,
-                            check_blocks_, st));
+    HIP_OK(launch_gray(src, gw, gh, gch, (int)gstep, s.d_gray, s.gpitch, st));
+    const int nl = (int)plan->levels.size();
+    if (s.levels_plan != plan && nl > 0) {
+        HIP_OK(hipMemcpyAsync(s.d_levels, plan->levels.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, st));
+        s.levels_plan = plan;
     }
-    if (profiling_) HIP_OK(hipEventRecord(ev_[4], st));
-    // read back the candidate list and the check results
-    HIP_OK(hipMemcpyAsync(h_meta_, d_meta_, sizeof(int), hipMemcpyDeviceToHost, st));
-    const size_t pre = std::min<size_t>(h_cap_, 4096);
-    HIP_OK(hipMemcpyAsync(h_cands_, d_cands_, sizeof(Candidate) * pre, hipMemcpyDeviceToHost, st));
-    if (checks) HIP_OK(hipMemcpyAsync(h_results_, d_results_, sizeof(CheckResult) * pre, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    const int count = nl > 0 ? h_meta_[0] : 0;
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
+    HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->ntiles, d_w_[0], s.d_resp, st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
+    HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_offsets, s.d_meta,
+                          s.d_cands, st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
+    s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
+    if (s.checks)
+        HIP_OK(launch_check(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), d_w_[1],
+                            d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_, st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
+    s.pre = std::min(std::min(s.h_cap, PREFETCH), s.cand_cap);
+    HIP_OK(hipMemcpyAsync(s.h_meta, s.d_meta, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (nl > 0) {
+        HIP_OK(hipMemcpyAsync(s.h_cands, s.d_cands, sizeof(Candidate) * s.pre, hipMemcpyDeviceToHost, st));
+        if (s.checks)
+            HIP_OK(hipMemcpyAsync(s.h_results, s.d_results, sizeof(CheckResult) * s.pre, hipMemcpyDeviceToHost, st));
+    }
+    s.plan = plan;
+    s.gw = gw;
+    s.gh = gh;
+    s.rescale = pre;
+    s.busy = true;
+    return 0;
+}
+
+// Waits for slot s and replays the reference's order-dependent decisions.
+int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
+{
+    faces.clear();
+    HIP_OK(hipStreamSynchronize(s.st));
+    s.busy = false;
+    last_slot_ = (int)(&s - slots_);
+    const std::vector<LevelDesc>& levels = s.plan->levels;
+    const int nl = (int)levels.size();
+    const int count = nl > 0 ? s.h_meta[0] : 0;
+    const bool checks = s.checks;
     std::vector<Candidate> big_c;
     std::vector<CheckResult> big_r;
-    const Candidate* cands = h_cands_;
-    const CheckResult* res = h_results_;
-    if ((size_t)count > pre) {
-        if ((size_t)count <= h_cap_) {
-            HIP_OK(hipMemcpy(h_cands_ + pre, d_cands_ + pre, sizeof(Candidate) * (count - pre), hipMemcpyDeviceToHost));
+    const Candidate* cands = s.h_cands;
+    const CheckResult* res = s.h_results;
+    if ((size_t)count > s.pre) {
+        if ((size_t)count <= s.h_cap) {
+            HIP_OK(hipMemcpy(s.h_cands + s.pre, s.d_cands + s.pre, sizeof(Candidate) * (count - s.pre),
+                             hipMemcpyDeviceToHost));
             if (checks)
-                HIP_OK(hipMemcpy(h_results_ + pre, d_results_ + pre, sizeof(CheckResult) * (count - pre),
+                HIP_OK(hipMemcpy(s.h_results + s.pre, s.d_results + s.pre, sizeof(CheckResult) * (count - s.pre),
                                  hipMemcpyDeviceToHost));
         } else {
             big_c.resize(count);
-            HIP_OK(hipMemcpy(big_c.data(), d_cands_, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(big_c.data(), s.d_cands, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
             cands = big_c.data();
             if (checks) {
                 big_r.resize(count);
-                HIP_OK(hipMemcpy(big_r.data(), d_results_, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
+                HIP_OK(hipMemcpy(big_r.data(), s.d_results, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
                 res = big_r.data();
             }
         }
     }
     if (profiling_) {
-        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&last_ms_[i == 0 ? 0 : i], ev_[i], ev_[i + 1]);
-        (void)hipEventElapsedTime(&last_ms_[4], ev_[0], ev_[4]);
+        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&last_ms_[i], s.ev[i], s.ev[i + 1]);
+        (void)hipEventElapsedTime(&last_ms_[4], s.ev[0], s.ev[4]);
+        for (int i = 0; i < 5; ++i) acc_ms_[i] += last_ms_[i];
+        acc_frames_++;
     }
     last_count_ = count;
-    last_levels_ = levels_;
+    last_levels_ = levels;
     last_cands_.assign(cands, cands + count);
     if (checks) last_results_.assign(res, res + count);
     else last_results_.clear();
@@ -645,8 +719,7 @@ int Detector::run_frame(const uint8_t* src, int w, int h, int ch, size_t step, s
     std::vector<int> points;
     int ci = 0;
     for (int l = 0; l < nl; ++l) {
-        const LevelDesc& L = levels_[l];
-        const float scale = L.scale;
+        const float scale = levels[l].scale;
         const float inv = 1.f / scale;
         points.clear();
         for (; ci < count && cands[ci].level == l; ++ci) {
@@ -717,13 +790,12 @@ int Detector::run_frame(const uint8_t* src, int w, int h, int ch, size_t step, s
         }
     }
     std::vector<Det> dets = post_process(rects, set_);
-    const bool rescale = gray_only_ && pre_scale_ > 0.f && ch == 1;
-    const float s = rescale ? (float)param.min_face_height / (float)net_[0].min_h : 1.f;   // :2783-2806
+    const float sc = s.rescale ? (float)param.min_face_height / (float)net_[0].min_h : 1.f;   // :2783-2806
     faces.reserve(dets.size());
     for (const Det& d : dets) {
         Rect r = d.r;
-        if (rescale)
-            r = Rect((int)((float)r.x * s), (int)((float)r.y * s), (int)((float)r.w * s), (int)((float)r.h * s));
+        if (s.rescale)
+            r = Rect((int)((float)r.x * sc), (int)((float)r.y * sc), (int)((float)r.w * sc), (int)((float)r.h * sc));
         ccnn_face f;
         f.x = r.x; f.y = r.y; f.width = r.w; f.height = r.h; f.score = d.score;
         f.gender = -1; f.smile = -1; f.glasses = -1;
@@ -732,26 +804,71 @@ int Detector::run_frame(const uint8_t* src, int w, int h, int ch, size_t step, s
     return (int)faces.size();
 }
 
+int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces)
+{
+    faces.clear();
+    const int r = prepare(img);
+    if (r <= 0) return r;
+    Slot& s = slots_[0];
+    if (s.busy) (void)hipStreamSynchronize(s.st);
+    if (enqueue(s, img, device_ptr) < 0) return -1;
+    return finish(s, faces);
+}
+
+int Detector::detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces)
+{
+    faces.assign(std::max(n, 0), std::vector<ccnn_face>());
+    int pending = -1;            // frame index in flight on slots_[pending % NSLOTS]
+    for (int i = 0; i < n; ++i) {
+        // the guards may re-initialise the detector: drain the frame in flight first
+        const bool guard = empty_ || !built_ || imgs[i].cols > param.max_image_width ||
+                           imgs[i].rows > param.max_image_height ||
+                           (gray_only_ && pre_scale_ > 0.f && imgs[i].channels != 1);
+        if (guard && pending >= 0) {
+            if (finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
+            pending = -1;
+        }
+        const int r = prepare(imgs[i]);
+        if (r < 0) {
+            faces[i].clear();
+            continue;
+        }
+        if (r == 0) continue;
+        Slot& s = slots_[i % NSLOTS];
+        if (s.busy) (void)hipStreamSynchronize(s.st);
+        if (enqueue(s, imgs[i], device_ptr) < 0) return -1;
+        if (pending >= 0 && finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
+        pending = i;
+    }
+    if (pending >= 0 && finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
+    return 0;
+}
+
 int Detector::gray_image(ccnn_image* img, int pipeline)
 {
-    if (!img || empty_ || !d_gray_ || last_gw_ <= 0) return -1;
+    if (!img || empty_ || last_slot_ < 0) return -1;
+    Slot& s = slots_[last_slot_];
+    if (!s.d_gray || s.gw <= 0) return -1;
+    if (s.busy) (void)hipStreamSynchronize(s.st);
     if (pipeline == CCNN_PIPELINE_CPU) {
-        h_gray_.resize((size_t)gpitch_ * last_gh_);
-        HIP_OK(hipMemcpy(h_gray_.data(), d_gray_, sizeof(float) * h_gray_.size(), hipMemcpyDeviceToHost));
+        h_gray_.resize((size_t)s.gpitch * s.gh);
+        HIP_OK(hipMemcpy(h_gray_.data(), s.d_gray, sizeof(float) * h_gray_.size(), hipMemcpyDeviceToHost));
         img->data = (unsigned char*)h_gray_.data();
     } else {
-        img->data = (unsigned char*)d_gray_;
+        img->data = (unsigned char*)s.d_gray;
     }
-    img->cols = last_gw_;
-    img->rows = last_gh_;
+    img->cols = s.gw;
+    img->rows = s.gh;
     img->channels = 1;
-    img->step = (size_t)gpitch_;
+    img->step = (size_t)s.gpitch;
     return 0;
 }
 
 int Detector::last_timing(float* ms) const
 {
     for (int i = 0; i < 5; ++i) ms[i] = last_ms_[i];
+    for (int i = 0; i < 5; ++i) ms[5 + i] = (float)acc_ms_[i];
+    ms[10] = (float)acc_frames_;
     return 0;
 }
 
@@ -790,6 +907,8 @@ int Detector::stage_check(const float* gray, int gw, int gh, int n, const float*
     LevelDesc* d_l = nullptr;
     Candidate* d_c = nullptr;
     CheckResult* d_r = nullptr;
+    int2* d_s2 = nullptr;
+    int2* d_s3 = nullptr;
     int* d_m = nullptr;
     std::vector<LevelDesc> lv(n);
     std::vector<Candidate> cd(n);
@@ -799,25 +918,21 @@ int Detector::stage_check(const float* gray, int gw, int gh, int n, const float*
         cd[i].level = i; cd[i].x = x[i]; cd[i].y = y[i]; cd[i].score = 0.f;
     }
     int rc = 0;
-    CheckParams P;
-    P.t2 = set_.t2; P.t3 = set_.t3; P.type_check = set_.type_check; P.min_neighbors = set_.min_neighbors;
-    P.adapt = set_.adapt; P.double_check = set_.double_check; P.equalize = set_.equalize;
-    P.reflection = set_.reflection;
     std::vector<CheckResult> out(n);
-    if (hipMalloc(&d_g, sizeof(float) * (size_t)gp * (gh + 1)) != hipSuccess ||
-        hipMalloc(&d_l, sizeof(LevelDesc) * n) != hipSuccess || hipMalloc(&d_c, sizeof(Candidate) * n) != hipSuccess ||
-        hipMalloc(&d_r, sizeof(CheckResult) * n) != hipSuccess || hipMalloc(&d_m, sizeof(int) * 4) != hipSuccess ||
+    hipStream_t st = slots_[0].st;
+    if (dalloc(d_g, (size_t)gp * (gh + 1)) < 0 || dalloc(d_l, n) < 0 || dalloc(d_c, n) < 0 || dalloc(d_r, n) < 0 ||
+        dalloc(d_s2, 2 * (size_t)n) < 0 || dalloc(d_s3, 2 * (size_t)n) < 0 || dalloc(d_m, 4) < 0 ||
         hipMemcpy2D(d_g, sizeof(float) * gp, gray, sizeof(float) * gw, sizeof(float) * gw, gh, hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemcpy(d_l, lv.data(), sizeof(LevelDesc) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_c, cd.data(), sizeof(Candidate) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_m, &n, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        launch_check(d_g, gp, gw, gh, d_l, d_c, d_m, P, d_w_[1], d_w_[2], d_r, std::min(n, 2048), stream_) !=
-            hipSuccess ||
-        hipStreamSynchronize(stream_) != hipSuccess ||
+        launch_check(d_g, gp, gw, gh, d_l, d_c, d_m, check_params(), d_w_[1], d_w_[2], d_s2, d_s3, d_r,
+                     std::min(n, 2048), st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(out.data(), d_r, sizeof(CheckResult) * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = -1;
-    (void)hipFree(d_g); (void)hipFree(d_l); (void)hipFree(d_c); (void)hipFree(d_r); (void)hipFree(d_m);
+    dfree(d_g); dfree(d_l); dfree(d_c); dfree(d_r); dfree(d_s2); dfree(d_s3); dfree(d_m);
     if (rc < 0) return -1;
     for (int i = 0; i < n; ++i) {
         float* o = res + 8 * i;

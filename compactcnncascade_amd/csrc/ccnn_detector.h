@@ -22,8 +22,11 @@ hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelD
 hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
                           int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
-                        const int* meta, const CheckParams& P, const float* W2, const float* W3,
-                        CheckResult* results, int nblocks, hipStream_t st);
+                        const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
+                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st);
+
+// Last HIP error code seen by the library on this thread (0 if none); optionally reset.
+int last_error(bool reset);
 
 // A model of the cascade in the reference's binary format, repacked in NetShape layout.
 struct NetWeights {
@@ -67,19 +70,65 @@ void merger(std::vector<Det>& out, std::vector<Det>& in, float threshold, bool d
 // Merger 0.5, min_num_detect filter, Merger 0.2 with deletion.
 std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s);
 
+// Pyramid plan of one gray-image size: levels in compute order (scl_max -> 0).
+struct Plan {
+    int gw = -1, gh = -1;
+    std::vector<LevelDesc> levels;
+    int ntiles = 0, nchunks = 0;
+    size_t resp_total = 0;
+};
+
+// Device and pinned-host state of one frame in flight. Two slots on two streams let the
+// GPU work on frame i+1 while the host replays the decisions of frame i.
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint8_t* d_img = nullptr;    size_t img_bytes = 0;
+    uint8_t* d_pre = nullptr;    size_t pre_bytes = 0;
+    float* d_gray = nullptr;     size_t gray_elems = 0;  int gpitch = 0;
+    float* d_resp = nullptr;     size_t resp_cap = 0;
+    LevelDesc* d_levels = nullptr;
+    int* d_counts = nullptr;     int* d_offsets = nullptr; size_t chunk_cap = 0;
+    int* d_meta = nullptr;
+    Candidate* d_cands = nullptr;
+    CheckResult* d_results = nullptr;
+    int2* d_s2 = nullptr;        int2* d_s3 = nullptr;
+    size_t cand_cap = 0;
+    int* h_meta = nullptr;
+    Candidate* h_cands = nullptr;
+    CheckResult* h_results = nullptr;
+    size_t h_cap = 0;
+    const Plan* levels_plan = nullptr;   // plan whose level table is in d_levels
+    // frame in flight
+    const Plan* plan = nullptr;
+    int gw = 0, gh = 0;
+    bool rescale = false, checks = false, busy = false;
+    size_t pre = 0;
+};
+
 class Detector {
 public:
+    static constexpr int NSLOTS = 2;
     Detector();
     ~Detector();
 
     int init(const ccnn_param& p);
     int detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces);
+    // Detection of n frames with the per-frame launches of consecutive frames pipelined
+    // over two streams. faces[i] receives frame i's detections; returns 0, or -1 on a
+    // HIP error or an uninitialised detector.
+    int detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
     int clear();
     bool empty() const { return empty_; }
 
     ccnn_param param;          // the API-level parameters (getParam)
     int gray_image(ccnn_image* img, int pipeline);
-    void set_profiling(bool on) { profiling_ = on; }
+    void set_profiling(bool on)
+    {
+        profiling_ = on;
+        for (auto& v : acc_ms_) v = 0.f;
+        acc_frames_ = 0;
+    }
     int last_timing(float* ms) const;
     int last_trace(int* n_levels, int* levels, int cap, float* cand, float* res) const;
 
@@ -90,47 +139,28 @@ public:
 private:
     int init_device();
     int build();                                  // Init chain after parameter changes
-    int ensure_buffers(int w, int h);
-    int plan_frame(int gw, int gh);
-    int run_frame(const uint8_t* d_src, int w, int h, int ch, size_t step, std::vector<ccnn_face>& faces);
-    void release();
+    int prepare(const ccnn_image& img);           // size guards and re-initialisation of Detect
+    const Plan* plan_for(int gw, int gh);
+    int ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& plan);
+    int enqueue(Slot& s, const ccnn_image& img, bool device_ptr);
+    int finish(Slot& s, std::vector<ccnn_face>& faces);
+    void release_slot(Slot& s);
+    CheckParams check_params() const;
 
     bool empty_ = true;
     bool built_ = false;
     bool gray_only_ = true;        // AdvancedParam::gray_image_only as forced by the API
     int device_ = 0;
-    hipStream_t stream_ = nullptr;
     Settings set_;
     NetWeights net_[3];
     float* d_w_[3] = {nullptr, nullptr, nullptr};
     std::vector<float> scales_;
     float pre_scale_ = 0.f;
     int ext_w_ = 47, ext_h_ = 51;
-
-    // buffers sized for max_image_size
-    int cap_w_ = 0, cap_h_ = 0;
-    uint8_t* d_img_ = nullptr;  size_t img_bytes_ = 0;
-    uint8_t* d_pre_ = nullptr;
-    float* d_gray_ = nullptr;   int gpitch_ = 0;
-    float* d_resp_ = nullptr;   size_t resp_cap_ = 0;
-    LevelDesc* d_levels_ = nullptr;
-    int* d_counts_ = nullptr;   int* d_offsets_ = nullptr; size_t chunk_cap_ = 0;
-    int* d_meta_ = nullptr;
-    Candidate* d_cands_ = nullptr;
-    CheckResult* d_results_ = nullptr;
-    size_t cand_cap_ = 0;
-    int* h_meta_ = nullptr;
-    Candidate* h_cands_ = nullptr;
-    CheckResult* h_results_ = nullptr;
-    size_t h_cap_ = 0;
+    std::vector<Plan*> plans_;
+    Slot slots_[NSLOTS];
+    int last_slot_ = -1;
     std::vector<float> h_gray_;
-
-    // plan of the current frame size
-    int plan_gw_ = -1, plan_gh_ = -1;
-    std::vector<LevelDesc> levels_;
-    int ntiles_ = 0, nchunks_ = 0;
-    size_t resp_total_ = 0;
-    int last_gw_ = 0, last_gh_ = 0;
 
     // last frame record
     int last_count_ = 0;
@@ -139,8 +169,9 @@ private:
     std::vector<LevelDesc> last_levels_;
 
     bool profiling_ = false;
-    hipEvent_t ev_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float last_ms_[5] = {0, 0, 0, 0, 0};
+    double acc_ms_[5] = {0, 0, 0, 0, 0};
+    int acc_frames_ = 0;
     int check_blocks_ = 2048;
 };
 

@@ -13,9 +13,10 @@
 //   k_count / k_scan / k_scatter
 //                  threshold + order-preserving compaction   (cnn_detector_v3.cpp:1918-1947)
 //                  of candidate windows (levels in check order, raster order inside a level)
-//   k_check        per-candidate crop, NN resize to 49x51,   (cnn_detector_v3.cpp:1485-1818,
-//                  u8, equalize, stage 2 + stage 3, retry     image_proc.cpp:357-368,720-800)
-//                  crop; one workgroup per candidate
+//   k_check2       per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
+//                  resize to 49x51, u8, equalize, stage 2     image_proc.cpp:357-368,720-800)
+//   k_check3       stage 3 on the mirrored crop when needed
+//   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1
 
 #include <hip/hip_runtime.h>
 #include "ccnn_internal.h"
@@ -568,123 +569,213 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict__ g, int gp, int gw, int gh,
-                                                         const LevelDesc* __restrict__ levels,
-                                                         const Candidate* __restrict__ cands,
-                                                         const int* __restrict__ meta, CheckParams P,
-                                                         const float* __restrict__ W2, const float* __restrict__ W3,
-                                                         CheckResult* __restrict__ results)
+// Crop geometry of one candidate (cnn_detector_v3.cpp:1491-1556): the window around the
+// stage-1 position, shrunk by 15% for the retry (mod 1), clamped to the image.
+struct Crop { int rx, ry, rcols, rrows; };
+
+__device__ __forceinline__ Crop crop_geometry(int px, int py, float scale, int mod, int gw, int gh)
+{
+    const float inv = 1.f / scale;
+    int x = (int)((float)(px - 6) * inv);      // (ext 11 + x offset 2) / 2
+    int y = (int)((float)(py - 5) * inv);      // 11 / 2
+    int width = (int)(36.f * inv);             // 23 + 11 + 2
+    int height = (int)(38.f * inv);            // 27 + 11
+    if (mod == 1) {
+        // x += w*0.15/2; w -= w*0.15 (the reference's build folds w - w*0.15f into w * 0.85f)
+        x = (int)((float)x + (float)width * 0.15f / 2.f);
+        width = (int)((float)width * 0.85f);
+        y = (int)((float)y + (float)height * 0.15f / 2.f);
+        height = (int)((float)height * 0.85f);
+    }
+    Crop c;
+    c.rx = min(max(x, 0), gw);
+    c.ry = min(max(y, 0), gh);
+    c.rcols = min(max(x + width, 0), gw) - c.rx;
+    c.rrows = min(max(y + height, 0), gh) - c.ry;
+    return c;
+}
+
+// Inclusive scan of one int per thread over a 256-thread block (4 waves of 64).
+__device__ __forceinline__ int block_incl_scan_256(int v, int* s_w)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int a = __shfl_up(v, off, 64);
+        if (lane >= off) v += a;
+    }
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += s_w[k];
+    __syncthreads();
+    return v + base;
+}
+
+// Crop of one (candidate, crop variant) into s.u8: NN resize to 49x51 (a copy at that
+// size), FloatToUChar (+0.5, truncate), then equalizeImage (image_proc.cpp:720-800):
+// cumulative histogram and LUT int((cf - cf[0]) * (255 / (cf[255] - cf[0])) + 0.5),
+// skipped if the histogram is flat.
+__device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr, bool equalize, ChkSmem& s,
+                             int* s_w)
+{
+    using namespace chk;
+    const int tid = threadIdx.x;
+    const bool copy = (cr.rcols == CW && cr.rrows == CH);
+    const uint32_t sclx = ((uint32_t)cr.rcols << 12) / CW + 1;
+    const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
+    s.hist[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < CW * CH; i += CHECK_THREADS) {
+        const int iy = i / CW, ix = i - iy * CW;
+        uint32_t px = ix, py = iy;
+        if (!copy) {
+            px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
+            py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
+        }
+        const float v = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+        const uint32_t u = (uint32_t)(v + 0.5f) & 255u;
+        s.u8[i] = (uint8_t)u;
+        if (equalize) atomicAdd(&s.hist[u], 1);
+    }
+    __syncthreads();
+    if (equalize) {
+        const int cf = block_incl_scan_256(s.hist[tid], s_w);
+        s.lut[tid] = cf;
+        __syncthreads();
+        const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
+        if (bmin != bmax) {
+            const float delt = 255.f / (bmax - bmin);
+            const int v = (int)(((float)cf - bmin) * delt + 0.5f);
+            __syncthreads();
+            s.lut[tid] = v;
+            __syncthreads();
+            for (int i = tid; i < CW * CH; i += CHECK_THREADS) s.u8[i] = (uint8_t)s.lut[s.u8[i]];
+        }
+        __syncthreads();
+    }
+}
+
+// Stage 2 of one (candidate, crop variant): crop preparation and the stage-2 network on
+// UCharToFloat(offset 2) (image_proc.cpp:416-427). grid = (workgroups, 2 crop variants).
+__global__ void __launch_bounds__(CHECK_THREADS) k_check2(const float* __restrict__ g, int gp, int gw, int gh,
+                                                          const LevelDesc* __restrict__ levels,
+                                                          const Candidate* __restrict__ cands,
+                                                          const int* __restrict__ meta, CheckParams P,
+                                                          const float* __restrict__ W2, int2* __restrict__ s2res)
 {
     using namespace chk;
     __shared__ ChkSmem s;
+    __shared__ int s_w[4];
     const int tid = threadIdx.x;
     const int total = meta[0];
+    const int mod = blockIdx.y;
+    if (mod == 1 && !P.double_check) return;
     for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
         const Candidate cd = cands[ci];
-        const float scale = levels[cd.level].scale;
-        const int pxp = cd.x * 4, pyp = cd.y * 4;          // shift_pattern = 4
-        const float inv = 1.f / scale;
-        const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
-        CheckResult res;
-        int n2 = 0, n3 = 0;
-        for (int mod = 0; mod < 2; ++mod) {
-            // crop geometry (cnn_detector_v3.cpp:1491-1556)
-            int x = (int)((float)(pxp - 6) * inv);
-            int y = (int)((float)(pyp - 5) * inv);
-            int width = (int)(36.f * inv);
-            int height = (int)(38.f * inv);
-            if (mod == 1) {
-                // x += w*0.15/2; w -= w*0.15 (folded by the reference's build into w * 0.85f)
-                x = (int)((float)x + (float)width * 0.15f / 2.f);
-                width = (int)((float)width * 0.85f);
-                y = (int)((float)y + (float)height * 0.15f / 2.f);
-                height = (int)((float)height * 0.85f);
-            }
-            const int rx = min(max(x, 0), gw);
-            const int ry = min(max(y, 0), gh);
-            const int rcols = min(max(x + width, 0), gw) - rx;
-            const int rrows = min(max(y + height, 0), gh) - ry;
-            // NN resize to 49x51 (or copy at that size) + FloatToUChar (+0.5, truncate)
-            const bool copy = (rcols == CW && rrows == CH);
-            const uint32_t sclx = ((uint32_t)rcols << 12) / CW + 1;
-            const uint32_t scly = ((uint32_t)rrows << 12) / CH + 1;
-            for (int i = tid; i < 256; i += CHECK_THREADS) s.hist[i] = 0;
-            __syncthreads();
-            for (int i = tid; i < CW * CH; i += CHECK_THREADS) {
-                const int iy = i / CW, ix = i - iy * CW;
-                uint32_t px = ix, py = iy;
-                if (!copy) {
-                    px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(rcols - 2));
-                    py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(rrows - 1));
-                }
-                const float v = g[(size_t)(ry + py) * gp + rx + px];
-                const uint32_t u = (uint32_t)(v + 0.5f);
-                s.u8[i] = (uint8_t)u;
-                if (P.equalize) atomicAdd(&s.hist[u & 255], 1);
-            }
-            __syncthreads();
-            if (P.equalize) {
-                // equalizeImage (image_proc.cpp:720-800, scalar branch)
-                if (tid == 0) {
-                    int acc = 0;
-                    for (int k = 0; k < 256; ++k) { acc += s.hist[k]; s.lut[k] = acc; }
-                }
-                __syncthreads();
-                const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
-                const bool apply = bmin != bmax;
-                if (apply) {
-                    const float delt = 255.f / (bmax - bmin);
-                    const int cf = s.lut[tid & 255];
-                    __syncthreads();
-                    if (tid < 256) s.lut[tid] = (int)(((float)cf - bmin) * delt + 0.5f);
-                    __syncthreads();
-                    for (int i = tid; i < CW * CH; i += CHECK_THREADS) s.u8[i] = (uint8_t)s.lut[s.u8[i]];
-                }
-                __syncthreads();
-            }
-            // stage 2 on UCharToFloat(offset 2) (image_proc.cpp:416-427)
-            for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
-                const int yy = i / IW, xx = i - yy * IW;
-                s.in[i] = (float)s.u8[yy * CW + xx + 2];
-            }
-            __syncthreads();
-            check_forward<Stage2>(W2, s);
-            n2++;
-            float m1 = -10.f, m2 = -10.f;
-            int k1 = 0, k2 = 0;
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, mod, gw, gh);
+        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w);
+        for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
+            const int yy = i / IW, xx = i - yy * IW;
+            s.in[i] = (float)s.u8[yy * CW + xx + 2];
+        }
+        __syncthreads();
+        check_forward<Stage2>(W2, s);
+        if (tid == 0) {
+            float m1 = -10.f;
+            int k1 = 0;
             for (int i = 0; i < RN; ++i) {
                 const float d = s.out[i];
                 m1 = (m1 < d) ? d : m1;
                 if (d > P.t2) k1++;
             }
-            bool skip3 = (P.type_check == 0 && k1 >= mn) || (P.type_check > 0 && k1 <= 0) ||
-                         (P.type_check == 2 && k1 < mn);
-            if (!skip3) {
-                // stage 3 on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0
-                __syncthreads();
-                for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
-                    const int yy = i / IW, xx = i - yy * IW;
-                    s.in[i] = (float)s.u8[yy * CW + (P.reflection ? 46 - xx : xx)];
-                }
-                __syncthreads();
-                check_forward<Stage3>(W3, s);
-                n3++;
-                for (int i = 0; i < RN; ++i) {
-                    const float d = s.out[i];
-                    m2 = (m2 < d) ? d : m2;
-                    if (d > P.t3) k2++;
-                }
+            s2res[(size_t)ci * 2 + mod] = make_int2(k1, __float_as_int(m1));
+        }
+        __syncthreads();
+    }
+}
+
+// Stage 3 of one (candidate, crop variant) on UCharToFloat_inv (mirror, image_proc.cpp:
+// 478-490) or offset 0, run unless the stage-2 outcome ends the check early
+// (cnn_detector_v3.cpp:1704-1706). grid = (workgroups, 2).
+__global__ void __launch_bounds__(CHECK_THREADS) k_check3(const float* __restrict__ g, int gp, int gw, int gh,
+                                                          const LevelDesc* __restrict__ levels,
+                                                          const Candidate* __restrict__ cands,
+                                                          const int* __restrict__ meta, CheckParams P,
+                                                          const float* __restrict__ W3,
+                                                          const int2* __restrict__ s2res, int2* __restrict__ s3res)
+{
+    using namespace chk;
+    __shared__ ChkSmem s;
+    __shared__ int s_w[4];
+    const int tid = threadIdx.x;
+    const int total = meta[0];
+    const int mod = blockIdx.y;
+    if (mod == 1 && !P.double_check) return;
+    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+        const size_t slot = (size_t)ci * 2 + mod;
+        const int k1 = s2res[slot].x;
+        const Candidate cd = cands[ci];
+        const float scale = levels[cd.level].scale;
+        const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
+        const bool skip3 = (P.type_check == 0 && k1 >= mn) || (P.type_check > 0 && k1 <= 0) ||
+                           (P.type_check == 2 && k1 < mn);
+        if (skip3) {
+            if (tid == 0) s3res[slot] = make_int2(-1, __float_as_int(-10.f));
+            continue;
+        }
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gw, gh);
+        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w);
+        for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
+            const int yy = i / IW, xx = i - yy * IW;
+            s.in[i] = (float)s.u8[yy * CW + (P.reflection ? 46 - xx : xx)];
+        }
+        __syncthreads();
+        check_forward<Stage3>(W3, s);
+        if (tid == 0) {
+            float m2 = -10.f;
+            int k2 = 0;
+            for (int i = 0; i < RN; ++i) {
+                const float d = s.out[i];
+                m2 = (m2 < d) ? d : m2;
+                if (d > P.t3) k2++;
             }
+            s3res[slot] = make_int2(k2, __float_as_int(m2));
+        }
+        __syncthreads();
+    }
+}
+
+// Decision of CPUCheckDetect (cnn_detector_v3.cpp:1709-1725, 1808-1811) from the stage-2/3
+// outcomes of both crop variants: mod 0, then mod 1 if mod 0 is rejected.
+__global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restrict__ levels,
+                                                      const Candidate* __restrict__ cands,
+                                                      const int* __restrict__ meta, CheckParams P,
+                                                      const int2* __restrict__ s2res, const int2* __restrict__ s3res,
+                                                      CheckResult* __restrict__ results)
+{
+    const int total = meta[0];
+    for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < total; ci += gridDim.x * blockDim.x) {
+        const float scale = levels[cands[ci].level].scale;
+        const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
+        CheckResult res;
+        res.n2 = 0;
+        res.n3 = 0;
+        for (int mod = 0; mod < 2; ++mod) {
+            const int2 a = s2res[(size_t)ci * 2 + mod];
+            const int2 b = s3res[(size_t)ci * 2 + mod];
+            const int k1 = a.x, k2 = b.x < 0 ? 0 : b.x;
+            const float m1 = __int_as_float(a.y), m2 = __int_as_float(b.y);
+            res.n2 += 1;
+            res.n3 += b.x < 0 ? 0 : 1;
             bool flag = false;
             if (P.type_check == 0) flag = (k1 >= mn || k2 >= mn);
             else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
             else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
             res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
-            res.n2 = n2; res.n3 = n3;
-            __syncthreads();
             if (flag || !P.double_check) break;
         }
-        if (tid == 0) results[ci] = res;
+        results[ci] = res;
     }
 }
 
@@ -740,10 +831,13 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
 }
 
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
-                        const int* meta, const CheckParams& P, const float* W2, const float* W3,
-                        CheckResult* results, int nblocks, hipStream_t st)
+                        const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
+                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st)
 {
-    k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, results);
+    dim3 grid(nblocks, 2);
+    k_check2<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, s2res);
+    k_check3<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W3, s2res, s3res);
+    k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results);
     return hipGetLastError();
 }
 

@@ -86,6 +86,15 @@ int ccnn_detect(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_ima
 /* As ccnn_detect, with img->data a device pointer already resident in GPU memory. */
 int ccnn_detect_device(ccnn_detector* d, ccnn_face* faces, int capacity, const ccnn_image* img);
 
+/* Detection of n frames (host images, or device pointers when device != 0), each with
+ * the semantics of one ccnn_detect call, the launches of consecutive frames pipelined
+ * over two GPU streams so that the host-side decisions of frame i overlap the GPU work
+ * of frame i+1. Frame i's faces go to faces[i*capacity ...] (at most capacity) and its
+ * face count to counts[i]. Returns 0, or -1. No counterpart in the reference, whose
+ * Detect takes one frame per call. */
+int ccnn_detect_batch(ccnn_detector* d, int n, const ccnn_image* imgs, int device, ccnn_face* faces,
+                      int capacity, int* counts);
+
 /* FaceDetector::Clear (:355-372): 0, or -1 if empty. */
 int ccnn_clear(ccnn_detector* d);
 /* FaceDetector::isEmpty (:374-377) */
@@ -137,9 +146,16 @@ int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int
 int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
                     float* cand, float* res);
 
-/* Per-kernel timing of the last ccnn_detect* call (HIP events on the detector's stream),
- * enabled by ccnn_set_profiling(d, 1). ms[0..4] = gray, stage1, compaction, check, total
- * device time; returns 0. */
+/* The last HIP error code the library met on the calling thread (0 if none), for callers
+ * that need to tell "no faces" from a failed call (Detect reports 0 faces either way, as
+ * the reference's does); reset != 0 clears it. */
+int ccnn_last_error(int reset);
+
+/* Per-kernel timing from HIP events recorded on the stream each frame runs on, enabled
+ * (and the totals reset) by ccnn_set_profiling(d, 1). ms[0..4] = gray (with upload and
+ * pre-resize), stage 1, compaction, stage-2/3 checks, all, for the last frame; ms[5..9]
+ * the same summed over the frames since profiling was enabled; ms[10] that frame count.
+ * ms must hold 11 floats. Returns 0. */
 void ccnn_set_profiling(ccnn_detector* d, int on);
 int ccnn_last_timing(const ccnn_detector* d, float* ms);
 

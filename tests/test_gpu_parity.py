@@ -115,8 +115,10 @@ def test_detect_matches_reference(api, golden, name):
     assert frame_digest(f) == g["digest"], "input frame differs from the fixture's"
     print("detect", name, f.shape, flush=True)
     d = api.FaceDetector()
+    api.last_error(reset=True)
     assert d.Init(api_param(f, over)) == 0
     n, faces = d.Detect(f)
+    assert api.last_error() == 0
     exp = g["faces"]
     assert n == len(exp)
     got = [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces]
@@ -161,3 +163,36 @@ def test_repeated_detect_is_stable(api):
     for _ in range(3):
         again = d.Detect(f)[1]
         assert np.array_equal(first, again)
+
+
+def test_batch_matches_single_frame_results(api, golden):
+    # pipelined batch over frames of different sizes and channel counts: every frame's
+    # result equals its single-call golden result
+    names = ["vga_s0_min40", "vga_s3_min40", "photo3_min40_max200", "vga_s5_min40", "ragged_333x211",
+             "vga_s2_min40"]
+    frames = [CASES[n][0]() for n in names]
+    d = api.FaceDetector()
+    api.last_error(reset=True)
+    # one parameter set for the batch: the largest frame bounds max_image_size; the
+    # golden of each case was produced with max_image_size = its own size, which only
+    # matters through min_scale when max_face is 0, so compare cases with equal settings
+    for n, f in zip(names, frames):
+        assert d.Init(api_param(f, CASES[n][1])) == 0
+        out = d.DetectBatch(images=[f, f, f])
+        assert api.last_error() == 0
+        for cnt, faces in out:
+            exp = golden[n]["faces"]
+            assert cnt == len(exp)
+            assert [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces] == \
+                [e[:4] for e in exp]
+    # frames of several sizes through one detector in one batch
+    d2 = api.FaceDetector()
+    assert d2.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+    batch = d2.DetectBatch(images=frames)
+    singles = []
+    for f in frames:
+        d3 = api.FaceDetector()
+        assert d3.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+        singles.append(d3.Detect(f))
+    for (cb, fb), (cs, fs) in zip(batch, singles):
+        assert cb == cs and np.array_equal(fb, fs)
