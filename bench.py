@@ -76,12 +76,41 @@ def cpu_baseline(frames, params, budget_s):
     p = default_params(w, h, **params)
     done = 0
     t0 = time.perf_counter()
-    while done < len(frames) and time.perf_counter() - t0 < budget_s:
-        ref.detect_loop(frames[done:done + 1], p, 1)
+    while time.perf_counter() - t0 < budget_s:
+        i = done % len(frames)
+        ref.detect_loop(frames[i:i + 1], p, 1)
         done += 1
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "frames/s", "cores": 1, "kind": "reference",
-            "sample": "%d of the workload's %dx%d frames, one thread, %.1f s" % (done, w, h, dt)}
+            "sample": "%d detections over the workload's first %d %dx%d frames (cycled), one thread, %.1f s"
+                      % (done, min(done, len(frames)), w, h, dt)}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes summarised under profiles/ (line `bytes_per_launch N`), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_hbm_traffic.txt" % (kernel, workload))
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch"):
+                    return int(float(line.split()[1]))
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(mib):
+    """A device-to-device copy of `mib` MiB, five times: the known-size kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the detection kernels."""
+    import torch
+    n = mib << 18
+    a = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 5}), flush=True)
 
 
 def main():
@@ -93,11 +122,17 @@ def main():
     ap.add_argument("--workload", default="fhd_min40_max240")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (frames shard across GPUs "
-              "as independent detectors; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here (frames would shard "
+              "across GPUs as independent detectors, one process per GPU). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import torch
     from compactcnncascade_amd import api
@@ -194,7 +229,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_stage1" if dominant == "stage1" else "k_check2", args.workload),
             "note": "FP32 VALU kernel on the exact path (separate mul/add, no packed FP32); its own ceiling "
                     "is %.1f T ops/s, frac of that %.3f" % (VALU_EXACT_PEAK_TOPS, achieved / VALU_EXACT_PEAK_TOPS),
         },

@@ -82,6 +82,8 @@ def load():
     L.ccnn_stage1.argtypes = [_P, _I, _I, _P]
     L.ccnn_stage_check.argtypes = [ctypes.POINTER(CParam), _P, _I, _I, _I, _P, _P, _P, _P]
     L.ccnn_last_trace.argtypes = [_P, ctypes.POINTER(_I), _P, _I, _P, _P]
+    L.ccnn_last_facial.argtypes = [_P, _P, _I]
+    L.ccnn_stage_landmarks.argtypes = [_P, _P]
     L.ccnn_set_profiling.argtypes = [_P, _I]
     L.ccnn_last_error.argtypes = [_I]
     L.ccnn_last_timing.argtypes = [_P, _P]
@@ -94,10 +96,15 @@ def _ptr(a):
 
 
 def _image(img):
-    img = np.ascontiguousarray(img, np.uint8)
-    h, w = img.shape[:2]
+    """ImageData over a uint8 array; a view whose rows are dense keeps its row step
+    (a crop of a wider frame), anything else is copied to a dense array first."""
+    img = np.asarray(img)
     ch = 1 if img.ndim == 2 else img.shape[2]
-    return img, CImage(w, h, ch, _ptr(img), w * ch)
+    rows_dense = img.dtype == np.uint8 and img.strides[1] == ch and (img.ndim == 2 or img.strides[2] == 1)
+    if not rows_dense or img.strides[0] < img.shape[1] * ch:
+        img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    return img, CImage(w, h, ch, _ptr(img), img.strides[0])
 
 
 class Param:
@@ -297,6 +304,15 @@ class FaceDetector:
         totals = {k: float(ms[5 + i]) for i, k in enumerate(keys)}
         return dict(last=last, totals=totals, frames=int(ms[10]))
 
+    def last_facial(self):
+        """Facial analysis of the last frame's candidates: int array [n, 4] of
+        (ok, gender, smile, glasses), in candidate order."""
+        n = self._L.ccnn_last_facial(self._h, None, 0)
+        out = np.zeros((max(n, 0), 4), np.int32)
+        if n > 0:
+            self._L.ccnn_last_facial(self._h, _ptr(out), n)
+        return out
+
     def last_trace(self):
         """Levels run and candidate windows of the last Detect, with their check results."""
         nl = _I()
@@ -341,6 +357,17 @@ def stage1(level):
     out = np.zeros((rh.value, rw.value), np.float32)
     if L.ccnn_stage1(_ptr(level), lw, lh, _ptr(out)) != 0:
         raise RuntimeError("ccnn_stage1 failed")
+    return out
+
+
+def stage_landmarks(crop):
+    """The facial-analysis network on one 51x49 FP32 crop: its 13 outputs."""
+    crop = np.ascontiguousarray(crop, np.float32)
+    if crop.shape != (51, 49):
+        raise ValueError("landmarks input is 49x51")
+    out = np.zeros(13, np.float32)
+    if load().ccnn_stage_landmarks(_ptr(crop), _ptr(out)) != 0:
+        raise RuntimeError("ccnn_stage_landmarks failed")
     return out
 
 

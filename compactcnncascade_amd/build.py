@@ -15,7 +15,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libccnn.so")
-MODELS = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin"]
+MODELS = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin", "cnn4landmarks_cntk.bin"]
 
 SOURCES = ["ccnn_kernels.hip", "ccnn_detector.cpp", "ccnn_api.cpp", "ccnn_stages.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -32,7 +32,7 @@ def gen_models(path):
         lines.append("static const unsigned char k_model%d[%d] = {%s};" % (i, len(data), body))
     lines.append("void embedded_model(int stage, const unsigned char** bytes, size_t* n)")
     lines.append("{")
-    for i in range(3):
+    for i in range(len(MODELS)):
         lines.append("    if (stage == %d) { *bytes = k_model%d; *n = sizeof(k_model%d); return; }" % (i, i, i))
     lines.append("    *bytes = nullptr; *n = 0;")
     lines.append("}")

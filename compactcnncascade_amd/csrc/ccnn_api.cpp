@@ -226,6 +226,12 @@ int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
     return d->det.last_trace(n_levels, levels, cap, cand, res);
 }
 
+int ccnn_last_facial(const ccnn_detector* d, int* out, int cap)
+{
+    if (!d) return -1;
+    return d->det.last_facial(out, cap);
+}
+
 int ccnn_last_error(int reset)
 {
     return ccnn::last_error(reset != 0);

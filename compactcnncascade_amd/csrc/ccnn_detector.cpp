@@ -90,11 +90,15 @@ int load_model(const unsigned char* bytes, size_t n, int stage, int index_output
     const int connect = r.get<int32_t>();
     const int hl_scale = r.get<int32_t>();
     if (r.bad || !max_pool || kw[1] * kh[1] != 9 || hl <= 0 || hl > 4096 || hl_scale <= 0 || connect <= 0 ||
-        hl % hl_scale != 0 || full_connect) {
+        hl % hl_scale != 0) {
         err = "This configuration cnn models is not supported!";
         return -1;
     }
-    std::vector<float> hl_w((size_t)hl * connect), hl_b(hl), hl_tw(hl / hl_scale), hl_bnw(hl_scale), hl_bnb(hl_scale);
+    // hidden-layer weights: `connect` per unit, or one per L3 map when fully connected
+    // (:331-354); the forward pass reads the first `connect` of them (:881)
+    const int nin = full_connect ? maps[2] : connect;
+    if (connect > nin) { err = "This configuration cnn models is not supported!"; return -1; }
+    std::vector<float> hl_w((size_t)hl * nin), hl_b(hl), hl_tw(hl / hl_scale), hl_bnw(hl_scale), hl_bnb(hl_scale);
     for (auto& v : hl_w) v = r.get<float>();
     for (auto& v : hl_b) v = r.get<float>();
     for (auto& v : hl_tw) v = r.get<float>();
@@ -106,15 +110,15 @@ int load_model(const unsigned char* bytes, size_t n, int stage, int index_output
     for (auto& v : ol_b) v = r.get<float>();
     const float ol_tw = r.get<float>();
     if (r.bad) { err = "Configuration file is truncated!"; return -1; }
-    int io = std::min(index_output, ol - 1);                    // :389-390
-    if (io < 0) { err = "Multi-output response maps (index_output < 0) are not supported!"; return -1; }
+    int io = std::min(index_output, ol - 1);                    // :395-396
     if (io == 0) af_scale = -af_scale;
 
     auto fill = [&](auto shape) -> int {
         using N = decltype(shape);
+        // index_output < 0 computes every output (the landmarks network); otherwise one
         if (maps[0] != N::M1 || maps[1] != N::M2 || maps[2] != N::M3 || kw[0] != N::K1 || kh[0] != N::K1 ||
             kw[2] != N::K3W || kh[2] != N::K3H || hl / hl_scale != N::HLG || hl_scale != N::HLS ||
-            connect != N::HLN) {
+            connect != N::HLN || (io < 0 ? ol != N::NOUT : N::NOUT != 1)) {
             err = "This configuration cnn models is not supported!";
             return -1;
         }
@@ -132,20 +136,24 @@ int load_model(const unsigned char* bytes, size_t n, int stage, int index_output
                 w[ob[l] + 2 * mm[l] + j] = lay[l][2][j];     // lrelu_w2
                 w[ob[l] + 3 * mm[l] + j] = lay[l][4][j];     // bn_b (bn_w is folded into w1/w2)
             }
-        std::copy(hl_w.begin(), hl_w.end(), w.begin() + N::oHW);
+        for (int u = 0; u < hl; ++u)
+            std::copy(hl_w.begin() + (size_t)u * nin, hl_w.begin() + (size_t)u * nin + connect,
+                      w.begin() + N::oHW + (size_t)u * N::HLN);
         std::copy(hl_b.begin(), hl_b.end(), w.begin() + N::oHB);
         std::copy(hl_tw.begin(), hl_tw.end(), w.begin() + N::oHT);
         std::copy(hl_bnw.begin(), hl_bnw.end(), w.begin() + N::oHBW);
         std::copy(hl_bnb.begin(), hl_bnb.end(), w.begin() + N::oHBB);
-        std::copy(ol_w.begin() + (size_t)io * hl, ol_w.begin() + (size_t)(io + 1) * hl, w.begin() + N::oOW);
-        w[N::oOB] = ol_b[io];
-        w[N::oOB + 1] = ol_tw;
-        w[N::oOB + 2] = af_scale;
+        const int o0 = io < 0 ? 0 : io;
+        std::copy(ol_w.begin() + (size_t)o0 * hl, ol_w.begin() + (size_t)(o0 + N::NOUT) * hl, w.begin() + N::oOW);
+        for (int k = 0; k < N::NOUT; ++k) w[N::oOB + k] = ol_b[o0 + k];
+        w[N::oOB + N::NOUT] = ol_tw;
+        w[N::oOB + N::NOUT + 1] = af_scale;
         return 0;
     };
     if (stage == 0) return fill(Stage1{});
     if (stage == 1) return fill(Stage2{});
-    return fill(Stage3{});
+    if (stage == 2) return fill(Stage3{});
+    return fill(Landmarks{});
 }
 
 // ---------------------------------------------------------------------------------
@@ -174,7 +182,8 @@ Settings effective_settings(const ccnn_param& p)
     case CCNN_PRECISION_ULTRA: s.type_check = 2; s.min_num_detect = 2; break;
     default: break;
     }
-    if (s.detect_mode == CCNN_DETECT_DISABLE) s.drop_detect = 0; // :961-964
+    s.facial = p.facial_analysis != 0;
+    if (s.detect_mode == CCNN_DETECT_DISABLE || s.facial) s.drop_detect = 0;   // :961-964
     return s;
 }
 
@@ -286,6 +295,7 @@ void merger(std::vector<Det>& out, std::vector<Det>& in, float threshold, bool d
                 mx.score = std::max(mx.score, it2.score);
                 mx.knn = std::max(mx.knn, it2.knn);
                 mx.scale = std::min(mx.scale, it2.scale);
+                mx.fd.insert(mx.fd.end(), it2.fd.begin(), it2.fd.end());   // :2911-2914
                 mx.num_detect++;
                 it2.checked = true;
             }
@@ -419,12 +429,14 @@ int Detector::build()
     for (auto& s : slots_)
         if (s.busy) { (void)hipStreamSynchronize(s.st); s.busy = false; }
     set_ = effective_settings(param);
-    for (int st = 0; st < 3; ++st) {
+    // the landmarks network only with facial analysis (cnn_detector_v3.cpp:792-810); it
+    // computes every output (AdvancedParam::index_output[3] = -1)
+    for (int st = 0; st < (set_.facial ? 4 : 3); ++st) {
         const unsigned char* b;
         size_t n;
         embedded_model(st, &b, &n);
         std::string err;
-        if (load_model(b, n, st, param.index_output[st], net_[st], err) < 0) {
+        if (load_model(b, n, st, st < 3 ? param.index_output[st] : -1, net_[st], err) < 0) {
             std::printf("[ccnn] %s\n", err.c_str());
             return -1;
         }
@@ -647,9 +659,13 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                           s.d_cands, st));
     if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
-    if (s.checks)
+    if (s.checks) {
         HIP_OK(launch_check(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), d_w_[1],
                             d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_, st));
+        if (set_.facial)
+            HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
+                                 fa_blocks_, st));
+    }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
     s.pre = std::min(std::min(s.h_cap, PREFETCH), s.cand_cap);
     HIP_OK(hipMemcpyAsync(s.h_meta, s.d_meta, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -766,6 +782,8 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
                     if (drop) continue;
                 }
                 const CheckResult& r = res[pi];
+                // facial analysis drops the window, without the retry (:1749)
+                if (r.flag && set_.facial && !r.fa.ok) continue;
                 if (r.flag) {
                     Det d;
                     d.r = nr;
@@ -773,6 +791,7 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
                     d.score = c.score + (float)r.k1 * std::max(-1.7159f, r.m1) + (float)r.k2 * std::max(-1.7159f, r.m2);
                     d.scale = scale;
                     d.knn = std::min(r.k1, r.k2);
+                    if (set_.facial) d.fd.push_back({r.fa.gender, r.fa.smile, r.fa.glasses});   // :1762-1765
                     rects.push_back(d);
                 }
             }
@@ -799,6 +818,17 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         ccnn_face f;
         f.x = r.x; f.y = r.y; f.width = r.w; f.height = r.h; f.score = d.score;
         f.gender = -1; f.smile = -1; f.glasses = -1;
+        if (!d.fd.empty()) {
+            // FaceDetector::Detect reports, per attribute, the element at size/2 of the
+            // entries sorted in descending order (CompactCNNLibAPI_v2.cpp:296-319)
+            int* dst[3] = {&f.gender, &f.smile, &f.glasses};
+            std::vector<int> v(d.fd.size());
+            for (int k = 0; k < 3; ++k) {
+                for (size_t i = 0; i < v.size(); ++i) v[i] = d.fd[i][k];
+                std::sort(v.begin(), v.end(), [](int a, int b) { return a > b; });
+                *dst[k] = v[v.size() / 2];
+            }
+        }
         faces.push_back(f);
     }
     return (int)faces.size();
@@ -894,6 +924,16 @@ int Detector::last_trace(int* n_levels, int* levels, int cap, float* cand, float
         o[4] = r.m1; o[5] = r.m2; o[6] = (float)r.n2; o[7] = (float)r.n3;
     }
     return last_count_;
+}
+
+int Detector::last_facial(int* out, int cap) const
+{
+    const int n = std::min(cap, (int)last_results_.size());
+    for (int i = 0; i < n && out; ++i) {
+        const FaResult& f = last_results_[i].fa;
+        out[4 * i] = f.ok; out[4 * i + 1] = f.gender; out[4 * i + 2] = f.smile; out[4 * i + 3] = f.glasses;
+    }
+    return (int)last_results_.size();
 }
 
 int Detector::stage_check(const float* gray, int gw, int gh, int n, const float* scale, const int* x, const int* y,

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,9 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, hipStream_t st);
+hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
+                         const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
+hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
 
 // Last HIP error code seen by the library on this thread (0 if none); optionally reset.
 int last_error(bool reset);
@@ -36,7 +40,8 @@ struct NetWeights {
 // Parses format v1.0-1.1 (cnn_simd_cntk.cpp:42-404) and checks the shape against the
 // stage's prototype. Returns 0 or -1 (message in err).
 int load_model(const unsigned char* bytes, size_t n, int stage, int index_output, NetWeights& out, std::string& err);
-// Embedded copies of the shipped models (cnn4face{1,2,3}_cntk.bin).
+// Embedded copies of the shipped models (stage 0..2: cnn4face{1,2,3}_cntk.bin,
+// 3: cnn4landmarks_cntk.bin).
 void embedded_model(int stage, const unsigned char** bytes, size_t* n);
 
 // Effective detector settings (FaceDetector::Init's conversion, CompactCNNLibAPI_v2.cpp:100-138,
@@ -45,6 +50,7 @@ struct Settings {
     int type_check = 1, min_num_detect = 1, adapt = 0, drop_detect = 1, merger = 1, double_check = 1;
     int equalize = 1, reflection = 1, detect_mode = 2, min_neighbors = 2;
     float t1 = -1.5f, t2 = -1.25f, t3 = -1.0f;
+    int facial = 0;
 };
 Settings effective_settings(const ccnn_param& p);
 
@@ -58,11 +64,13 @@ struct Rect {
     Rect() = default;
     Rect(int x_, int y_, int w_, int h_) : x(x_), y(y_), w(w_), h(h_), x2(x_ + w_), y2(y_ + h_) {}
 };
+// Detection (cnn_detector_v3.h:185-216); fd: the facial_data entries {gender, smile, glasses}
 struct Det {
     Rect r;
     float score = 0.f, scale = 0.f;
     int knn = 0, num_detect = 0;
     bool checked = false;
+    std::vector<std::array<int, 3>> fd;
 };
 // Merger (cnn_detector_v3.cpp:2814-2936)
 void merger(std::vector<Det>& out, std::vector<Det>& in, float threshold, bool del);
@@ -135,6 +143,8 @@ public:
     // stage-level access (tests)
     int stage_check(const float* gray, int gw, int gh, int n, const float* scale, const int* x, const int* y,
                     float* res);
+    // facial analysis results of the last frame's candidates: {ok, gender, smile, glasses}
+    int last_facial(int* out, int cap) const;
 
 private:
     int init_device();
@@ -152,8 +162,8 @@ private:
     bool gray_only_ = true;        // AdvancedParam::gray_image_only as forced by the API
     int device_ = 0;
     Settings set_;
-    NetWeights net_[3];
-    float* d_w_[3] = {nullptr, nullptr, nullptr};
+    NetWeights net_[4];
+    float* d_w_[4] = {nullptr, nullptr, nullptr, nullptr};   // stages 1-3, landmarks
     std::vector<float> scales_;
     float pre_scale_ = 0.f;
     int ext_w_ = 47, ext_h_ = 51;
@@ -173,6 +183,7 @@ private:
     double acc_ms_[5] = {0, 0, 0, 0, 0};
     int acc_frames_ = 0;
     int check_blocks_ = 2048;
+    int fa_blocks_ = 256;          // one 118 KB-LDS workgroup per CU
 };
 
 }  // namespace ccnn

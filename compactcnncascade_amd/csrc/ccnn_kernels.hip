@@ -75,21 +75,63 @@ __device__ __forceinline__ float level_px(const float* __restrict__ g, int gp, i
 // ---------------------------------------------------------------------------------
 // Gray conversion
 // ---------------------------------------------------------------------------------
+__device__ __forceinline__ float gray_of(float B, float G, float R)
+{
+    // 0.114 B + 0.587 G + 0.299 R, summed left to right (image_proc.cpp:99-102)
+    return 0.114f * B + 0.587f * G + 0.299f * R;
+}
+
+__device__ __forceinline__ float ub(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
+
+// One thread per 4 pixels of a row (a quad), rows flattened over the grid: 4-byte loads
+// of the frame, one 16-byte store of gray. Needs a 4-byte aligned frame and row step and
+// a gray pitch that is a multiple of 4 floats; the last quad of a row may be partial.
 __global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, int w, int h, int ch, int step,
-                                              float* __restrict__ gray, int gp)
+                                              float* __restrict__ gray, int gp, int qw)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= qw * h) return;
+    const int y = q / qw;
+    const int x = (q - y * qw) * 4;
+    const uint8_t* s = img + (size_t)y * step;
+    float* d = gray + (size_t)y * gp + x;
+    if (x + 4 <= w) {
+        float4 v;
+        if (ch == 3) {
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(s + (size_t)x * 3);
+            const uint32_t a = p[0], b = p[1], c = p[2];
+            v.x = gray_of(ub(a, 0), ub(a, 1), ub(a, 2));
+            v.y = gray_of(ub(a, 3), ub(b, 0), ub(b, 1));
+            v.z = gray_of(ub(b, 2), ub(b, 3), ub(c, 0));
+            v.w = gray_of(ub(c, 1), ub(c, 2), ub(c, 3));
+        } else if (ch == 4) {
+            const uint4 p = *reinterpret_cast<const uint4*>(s + (size_t)x * 4);
+            v.x = gray_of(ub(p.x, 0), ub(p.x, 1), ub(p.x, 2));
+            v.y = gray_of(ub(p.y, 0), ub(p.y, 1), ub(p.y, 2));
+            v.z = gray_of(ub(p.z, 0), ub(p.z, 1), ub(p.z, 2));
+            v.w = gray_of(ub(p.w, 0), ub(p.w, 1), ub(p.w, 2));
+        } else {
+            const uint32_t a = *reinterpret_cast<const uint32_t*>(s + x);
+            v.x = ub(a, 0); v.y = ub(a, 1); v.z = ub(a, 2); v.w = ub(a, 3);
+        }
+        *reinterpret_cast<float4*>(d) = v;
+    } else {
+        for (int i = 0; x + i < w; ++i) {
+            const uint8_t* t = s + (size_t)(x + i) * ch;
+            d[i] = ch == 1 ? (float)t[0] : gray_of((float)t[0], (float)t[1], (float)t[2]);
+        }
+    }
+}
+
+// Any alignment: one thread per pixel, byte loads.
+__global__ void __launch_bounds__(256) k_gray_bytes(const uint8_t* __restrict__ img, int w, int h, int ch,
+                                                    int step, float* __restrict__ gray, int gp)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= w || y >= h) return;
     const uint8_t* s = img + (size_t)y * step + (size_t)x * ch;
-    float v;
-    if (ch == 1) {
-        v = (float)s[0];
-    } else {
-        const float B = (float)s[0], G = (float)s[1], R = (float)s[2];
-        v = 0.114f * B + 0.587f * G + 0.299f * R;
-    }
-    gray[(size_t)y * gp + x] = v;
+    gray[(size_t)y * gp + x] = ch == 1 ? (float)s[0] : gray_of((float)s[0], (float)s[1], (float)s[2]);
 }
 
 // 8-bit bilinear resize (ImageResizer::BilinearInterpolation for Image_8u, scalar tail,
@@ -174,11 +216,27 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
     const int x0 = 4 * ox0, y0 = 4 * oy0;
 
     // input window (clamped at the level border; clamped samples only feed outputs
-    // outside the response map, which are not stored)
-    for (int i = tid; i < IW * IH; i += S1_THREADS) {
-        const int iy = i / IW, ix = i - iy * IW;
-        const int lx = min(x0 + ix, L.lw - 1), ly = min(y0 + iy, L.lh - 1);
-        s_in[i] = level_px(g, gp, gw, gh, L, lx, ly);
+    // outside the response map, which are not stored). Samples go in batches of 8 per
+    // thread so that their gray-image loads are in flight together.
+    {
+        constexpr int NS = IW * IH, PER = (NS + S1_THREADS - 1) / S1_THREADS, B = 8;
+#pragma unroll 1
+        for (int b0 = 0; b0 < PER; b0 += B) {
+            float v[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int i = tid + (b0 + k) * S1_THREADS;
+                const int ii = min(i, NS - 1);
+                const int iy = ii / IW, ix = ii - iy * IW;
+                const int lx = min(x0 + ix, L.lw - 1), ly = min(y0 + iy, L.lh - 1);
+                v[k] = level_px(g, gp, gw, gh, L, lx, ly);
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int i = tid + (b0 + k) * S1_THREADS;
+                if (i < NS) s_in[i] = v[k];
+            }
+        }
     }
     __syncthreads();
 
@@ -775,8 +833,210 @@ __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restric
             res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
             if (flag || !P.double_check) break;
         }
+        res.fa.ok = 1;
+        res.fa.gender = res.fa.smile = res.fa.glasses = -1;
         results[ci] = res;
     }
+}
+
+// ---------------------------------------------------------------------------------
+// Facial analysis (cnn_detector_v3.cpp:1731-1756): the landmarks network on the 49x51
+// FP32 crop (the nearest-neighbour resize before the u8 conversion) of every window the
+// checks accept. One workgroup of 512 threads per accepted candidate.
+// ---------------------------------------------------------------------------------
+namespace fa {
+using N = Landmarks;
+constexpr int W = 49, H = 51;                           // input
+constexpr int P1W = (W - 3) / 2, P1H = (H - 3) / 2;     // 23 x 24 pooled L1
+constexpr int P1N = P1W * P1H;
+constexpr int P2W = (P1W - 2) / 2, P2H = (P1H - 2) / 2; // 10 x 11 pooled L2
+constexpr int P2N = P2W * P2H;
+constexpr int G1 = 6;                                   // L1 maps per work item
+static_assert(P2W == N::K3W && P2H == N::K3H, "one L3 value per map");
+}  // namespace fa
+
+struct FaSmem {
+    float in[fa::H * fa::W];
+    float a[Landmarks::M1 * fa::P1N];    // pooled L1, later pooled L2
+    float b[Landmarks::M1 * fa::P1N];    // L1 sums, later L2 sums
+    float a3[Landmarks::M3];
+    float h[Landmarks::HL];
+    float out[16];
+};
+
+// The landmarks network on s.in; all NOUT outputs to s.out. Layer arithmetic as in
+// check_forward: conv_4x4 / conv_3x3 chains, lrelu_bn(_max), add / add2 sums; L3 is
+// conv_11x10 (cnnpp_cplusplus.cpp:352-380), a chain that starts from 0.f; the hidden
+// layer is mulCN_add_tanhW over all 96 maps; the outputs are mulC / mulC1_add / tanhW.
+__device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
+{
+    using namespace fa;
+    const int tid = threadIdx.x;
+    for (int it = tid; it < (N::M1 / G1) * P1N; it += FA_THREADS) {
+        const int g = it / P1N, p = it - g * P1N;
+        const int py = p / P1W, px = p - py * P1W;
+        float win[5][5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) win[r][c] = s.in[(2 * py + r) * fa::W + 2 * px + c];
+#pragma unroll
+        for (int mm = 0; mm < G1; ++mm) {
+            const int m = g * G1 + mm;
+            const float* k = W + N::oK1 + m * 16;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 16; ++j) acc = acc + win[dy + (j >> 2)][dx + (j & 3)] * k[j];
+                a[q] = lrelu_bn(acc, W + N::oB1, m, N::M1);
+            }
+            s.a[m * P1N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+    }
+    __syncthreads();
+    for (int it = tid; it < N::M1 * P1N; it += FA_THREADS) {
+        const int i = it / P1N, p = it - i * P1N;
+        float v;
+        if (i == 0) v = s.a[p] + s.a[P1N + p];
+        else if (i == N::M1 - 1) v = s.a[(N::M1 - 2) * P1N + p] + s.a[(N::M1 - 1) * P1N + p];
+        else v = s.a[(i - 1) * P1N + p] + s.a[i * P1N + p] + s.a[(i + 1) * P1N + p];
+        s.b[it] = v;
+    }
+    __syncthreads();
+    for (int it = tid; it < N::M1 * P2N; it += FA_THREADS) {
+        const int i = it / P2N, p = it - i * P2N;
+        const int qy = p / P2W, qx = p - qy * P2W;
+        const float* src = s.b + i * P1N + (2 * qy) * P1W + 2 * qx;
+        float win[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int m = 2 * i + half;
+            const float* k = W + N::oK2 + m * 9;
+            float a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dx = q & 1, dy = q >> 1;
+                float acc = win[dy][dx] * k[0];
+#pragma unroll
+                for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
+                a[q] = lrelu_bn(acc, W + N::oB2, m, N::M2);
+            }
+            s.a[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
+    }
+    __syncthreads();
+    for (int it = tid; it < N::M2 * P2N; it += FA_THREADS) {
+        const int i = it / P2N, p = it - i * P2N;
+        float v;
+        if (i == 0) v = s.a[p] + s.a[P2N + p];
+        else if (i == N::M2 - 1) v = s.a[(N::M2 - 2) * P2N + p] + s.a[(N::M2 - 1) * P2N + p];
+        else v = s.a[(i - 1) * P2N + p] + s.a[i * P2N + p] + s.a[(i + 1) * P2N + p];
+        s.b[it] = v;
+    }
+    __syncthreads();
+    if (tid < N::M3) {
+        const int m = tid, i = m >> 1;
+        const float* src = s.b + i * P2N;
+        const float* k = W + N::oK3 + m * N::K3W * N::K3H;
+        float d = 0.f;
+        for (int j = 0; j < N::K3W * N::K3H; ++j) d = d + src[j] * k[j];
+        s.a3[m] = lrelu_bn(d, W + N::oB3, m, N::M3);
+    }
+    __syncthreads();
+    for (int u = tid; u < N::HL; u += FA_THREADS) {
+        const int gi = u / N::HLS, tt = u - gi * N::HLS;
+        const float* w = W + N::oHW + (size_t)u * N::HLN;
+        float c = W[N::oHB + u];
+        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[gi + j] * w[j];
+        c = c * W[N::oHT + gi];
+        c = rtanh(c);
+        c = (c + 1.f) * 0.5f;
+        s.h[u] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+    }
+    __syncthreads();
+    if (tid < N::NOUT) {
+        const float* w = W + N::oOW + (size_t)tid * N::HL;
+        float o = 0.f;
+        for (int gi = 0; gi < N::HLG; ++gi)
+            for (int tt = 0; tt < N::HLS; ++tt) {
+                const float wv = w[tt * N::HLG + gi];
+                if (gi == 0 && tt == 0) o = s.h[0] * wv;
+                else o = s.h[N::HLS * gi + tt] * wv + o;
+            }
+        float c = o + W[N::oOB + tid];
+        c = c * W[N::oOB + N::NOUT];
+        c = rtanh(c);
+        const float af = W[N::oOB + N::NOUT + 1];
+        s.out[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
+    }
+    __syncthreads();
+}
+
+// Facial analysis of every accepted candidate, on the crop of its final crop variant.
+__global__ void __launch_bounds__(FA_THREADS) k_facial(const float* __restrict__ g, int gp, int gw, int gh,
+                                                       const LevelDesc* __restrict__ levels,
+                                                       const Candidate* __restrict__ cands,
+                                                       const int* __restrict__ meta, const float* __restrict__ W,
+                                                       CheckResult* __restrict__ results)
+{
+    using namespace fa;
+    __shared__ FaSmem s;
+    const int tid = threadIdx.x;
+    const int total = meta[0];
+    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+        if (!results[ci].flag) continue;                  // uniform over the workgroup
+        const Candidate cd = cands[ci];
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, results[ci].mod, gw, gh);
+        // nearest-neighbour resize of the crop to 49x51, a copy at that size
+        // (image_resize.cpp:328-392, 564-584)
+        const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
+        const uint32_t sclx = ((uint32_t)cr.rcols << 12) / fa::W + 1;
+        const uint32_t scly = ((uint32_t)cr.rrows << 12) / fa::H + 1;
+        for (int i = tid; i < fa::W * fa::H; i += FA_THREADS) {
+            const int iy = i / fa::W, ix = i - iy * fa::W;
+            uint32_t px = ix, py = iy;
+            if (!copy) {
+                px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
+                py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
+            }
+            s.in[i] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+        }
+        __syncthreads();
+        landmarks_forward(W, s);
+        if (tid == 0) {
+            int zero = 0;
+            for (int t = 0; t < 5; ++t) {
+                const int lx = (int)(s.out[2 * t] * (float)cr.rcols);
+                const int ly = (int)(s.out[2 * t + 1] * (float)cr.rrows);
+                if (lx < (cr.rcols >> 1) && ly < (cr.rrows >> 1)) zero++;
+            }
+            FaResult f;
+            f.ok = zero != 5;
+            f.gender = (int)(s.out[10] + 0.5f);
+            f.smile = (int)(s.out[11] + 0.5f);
+            f.glasses = (int)(s.out[12] + 0.5f);
+            results[ci].fa = f;
+        }
+        __syncthreads();
+    }
+}
+
+// The landmarks network on one 49x51 input (stage-level entry point).
+__global__ void __launch_bounds__(FA_THREADS) k_landmarks(const float* __restrict__ in,
+                                                          const float* __restrict__ W, float* __restrict__ out)
+{
+    __shared__ FaSmem s;
+    for (int i = threadIdx.x; i < fa::W * fa::H; i += FA_THREADS) s.in[i] = in[i];
+    __syncthreads();
+    landmarks_forward(W, s);
+    if (threadIdx.x < Landmarks::NOUT) out[threadIdx.x] = s.out[threadIdx.x];
 }
 
 }  // namespace ccnn
@@ -788,8 +1048,16 @@ namespace ccnn {
 
 hipError_t launch_gray(const uint8_t* img, int w, int h, int ch, int step, float* gray, int gp, hipStream_t st)
 {
-    dim3 grid((w + 255) / 256, h);
-    k_gray<<<grid, 256, 0, st>>>(img, w, h, ch, step, gray, gp);
+    const bool aligned = ((uintptr_t)img % (ch == 4 ? 16 : 4)) == 0 && step % (ch == 4 ? 16 : 4) == 0 &&
+                         ((uintptr_t)gray % 16) == 0 && gp % 4 == 0;
+    if (aligned) {
+        const int qw = (w + 3) / 4;
+        const long long quads = (long long)qw * h;
+        k_gray<<<(unsigned)((quads + 255) / 256), 256, 0, st>>>(img, w, h, ch, step, gray, gp, qw);
+    } else {
+        dim3 grid((w + 255) / 256, h);
+        k_gray_bytes<<<grid, 256, 0, st>>>(img, w, h, ch, step, gray, gp);
+    }
     return hipGetLastError();
 }
 
@@ -827,6 +1095,19 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
     k_count<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, counts);
     k_scan<<<1, 1024, 0, st>>>(counts, nchunks, offsets, meta);
     k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, offsets, cands);
+    return hipGetLastError();
+}
+
+hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
+                         const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st)
+{
+    k_facial<<<nblocks, FA_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, W, results);
+    return hipGetLastError();
+}
+
+hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st)
+{
+    k_landmarks<<<1, FA_THREADS, 0, st>>>(in, W, out);
     return hipGetLastError();
 }
 

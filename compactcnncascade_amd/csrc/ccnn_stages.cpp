@@ -35,13 +35,16 @@ int ccnn_stage_gray(const ccnn_image* img, float* out)
     if (img->channels != 1 && img->channels != 3 && img->channels != 4) return -1;
     if (!have_device()) return -1;
     const size_t bytes = img->step * (size_t)(img->rows - 1) + (size_t)img->cols * img->channels;
+    // the detector's gray layout: pitch rounded up to 64 floats
+    const int gp = (img->cols + 63) / 64 * 64;
     DevBuf src, dst;
-    if (!src.alloc(bytes) || !dst.alloc(sizeof(float) * (size_t)img->cols * img->rows)) return -1;
+    if (!src.alloc(bytes) || !dst.alloc(sizeof(float) * (size_t)gp * img->rows)) return -1;
     if (hipMemcpy(src.p, img->data, bytes, hipMemcpyHostToDevice) != hipSuccess) return -1;
     if (ccnn::launch_gray(src.as<uint8_t>(), img->cols, img->rows, img->channels, (int)img->step, dst.as<float>(),
-                          img->cols, nullptr) != hipSuccess)
+                          gp, nullptr) != hipSuccess)
         return -1;
-    if (hipMemcpy(out, dst.p, sizeof(float) * (size_t)img->cols * img->rows, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy2D(out, sizeof(float) * (size_t)img->cols, dst.p, sizeof(float) * (size_t)gp,
+                    sizeof(float) * (size_t)img->cols, img->rows, hipMemcpyDeviceToHost) != hipSuccess)
         return -1;
     return 0;
 }
@@ -70,6 +73,28 @@ int ccnn_stage1_size(int lw, int lh, int* rw, int* rh)
     *rw = (c2 >> 1) - (ccnn::Stage1::K3W - 1);
     *rh = (r2 >> 1) - (ccnn::Stage1::K3H - 1);
     return (*rw > 0 && *rh > 0) ? 0 : -1;
+}
+
+int ccnn_stage_landmarks(const float* in, float* out)
+{
+    if (!in || !out) return -1;
+    if (!have_device()) return -1;
+    ccnn::NetWeights net;
+    const unsigned char* b;
+    size_t n;
+    ccnn::embedded_model(3, &b, &n);
+    std::string err;
+    if (ccnn::load_model(b, n, 3, -1, net, err) < 0) return -1;
+    DevBuf i, w, o;
+    if (!i.alloc(sizeof(float) * 49 * 51) || !w.alloc(sizeof(float) * net.w.size()) ||
+        !o.alloc(sizeof(float) * ccnn::Landmarks::NOUT))
+        return -1;
+    if (hipMemcpy(i.p, in, sizeof(float) * 49 * 51, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(w.p, net.w.data(), sizeof(float) * net.w.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    if (ccnn::launch_landmarks(i.as<float>(), w.as<float>(), o.as<float>(), nullptr) != hipSuccess) return -1;
+    if (hipMemcpy(out, o.p, sizeof(float) * ccnn::Landmarks::NOUT, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
 }
 
 int ccnn_stage1(const float* level, int lw, int lh, float* out)

@@ -138,6 +138,12 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out);
 int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int n,
                      const float* scale, const int* x, const int* y, float* res);
 
+/* The facial-analysis network (cnn4landmarks_cntk.bin, run by CPUCheckDetect on accepted
+ * windows when Param::facial_analysis is set, cnn_detector_v3.cpp:1731-1756) on one 49x51
+ * FP32 input (row-major, 49 columns); out: its 13 outputs (landmarks x0,y0..x4,y4 as
+ * fractions of the crop, gender, smile, glasses). */
+int ccnn_stage_landmarks(const float* in, float* out);
+
 /* Record of the last ccnn_detect* call on d: the levels run (compute order) and the
  * candidate windows with their check results. Returns the candidate count; levels[6*l..]
  * = {scale index, lw, lh, rw, rh, mode}, n_levels receives the level count;
@@ -145,6 +151,11 @@ int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int
  * NULL to query the counts. */
 int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
                     float* cand, float* res);
+
+/* Facial analysis of the last ccnn_detect* call's candidates (same order as
+ * ccnn_last_trace): out[4*i..] = {ok, gender, smile, glasses}; ok 0 marks a window the
+ * checks accepted and the landmarks rejected. Returns the candidate count; out may be NULL. */
+int ccnn_last_facial(const ccnn_detector* d, int* out, int cap);
 
 /* The last HIP error code the library met on the calling thread (0 if none), for callers
  * that need to tell "no faces" from a failed call (Detect reports 0 faces either way, as

@@ -17,8 +17,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 MODEL_DIR = os.path.join(os.path.dirname(HERE), "compactcnncascade_amd", "models")
-MODEL_FILES = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin"]
-INDEX_OUTPUT = [1, 0, 0]   # FaceDetector::Param::index_output defaults
+MODEL_FILES = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin", "cnn4landmarks_cntk.bin"]
+# FaceDetector::Param::index_output defaults; the landmarks network computes every output
+# (AdvancedParam::index_output[3] = -1, cnn_detector_v3.h:170-173)
+INDEX_OUTPUT = [1, 0, 0, -1]
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -43,7 +45,7 @@ class OcParams(ctypes.Structure):
     _fields_ = [("max_w", _I), ("max_h", _I), ("min_face", _I), ("max_face", _I),
                 ("min_neighbors", _I), ("precision", _I), ("drop_detect", _I), ("equalize", _I),
                 ("reflection", _I), ("detect_mode", _I), ("scale_factor", _F),
-                ("t1", _F), ("t2", _F), ("t3", _F)]
+                ("t1", _F), ("t2", _F), ("t3", _F), ("facial_analysis", _I)]
 
 
 class OcTrace(ctypes.Structure):
@@ -61,7 +63,7 @@ def default_params(w, h, **kw):
     frame size as max_image_size."""
     p = dict(max_w=w, max_h=h, min_face=80, max_face=0, min_neighbors=2, precision=2,
              drop_detect=1, equalize=1, reflection=1, detect_mode=2, scale_factor=1.15,
-             t1=-1.5, t2=-1.25, t3=-1.0)
+             t1=-1.5, t2=-1.25, t3=-1.0, facial_analysis=0)
     p.update(kw)
     return p
 
@@ -86,14 +88,14 @@ class Oracle:
         L.oc_resize_f32.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I]
         L.oc_resize_u8.argtypes = [_P, _I, _I, _I, _P, _I, _I, _I, _I]
         L.oc_equalize.argtypes = [_P, _I, _I, _I]
-        L.oc_check_prep.argtypes = [_P, _I, _I, _I, _I, _I, _P, _P, _P]
+        L.oc_check_prep.argtypes = [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P]
         L.oc_sort_desc.argtypes = [_P, _P, _I]
-        L.oc_detect.argtypes = [ctypes.POINTER(OcModel)] * 3 + [ctypes.POINTER(OcParams), _P, _I, _I, _I, _I,
+        L.oc_detect.argtypes = [ctypes.POINTER(OcModel)] * 4 + [ctypes.POINTER(OcParams), _P, _I, _I, _I, _I,
                                                                 _P, _I, ctypes.POINTER(OcTrace)]
         L.oc_check_candidate.argtypes = [ctypes.POINTER(OcModel)] * 2 + [ctypes.POINTER(OcParams), _P, _I, _I, _I,
                                                                          _F, _I, _I, _P]
         self.models = []
-        for i in range(3):
+        for i in range(4):
             m = OcModel()
             b = model_bytes(i)
             if L.oc_model_load(ctypes.byref(m), b, len(b), INDEX_OUTPUT[i]) != 0:
@@ -106,10 +108,12 @@ class Oracle:
         return ow.value, oh.value
 
     def forward(self, stage, img):
+        """Output map [oh, ow]; for the landmarks network (stage 3) all outputs, [ol, oh, ow]."""
         img = np.ascontiguousarray(img, np.float32)
         h, w = img.shape
         ow, oh = self.output_size(stage, w, h)
-        out = np.zeros((oh, ow), np.float32)
+        m = self.models[stage]
+        out = np.zeros((m.ol, oh, ow) if m.index_output < 0 else (oh, ow), np.float32)
         if self.lib.oc_cnn_forward(ctypes.byref(self.models[stage]), _ptr(img), w, h, w, _ptr(out)) != 0:
             raise ValueError("image below the model minimum")
         return out
@@ -148,8 +152,9 @@ class Oracle:
         u8 = np.zeros((51, 49), np.uint8)
         s2 = np.zeros((51, 47), np.float32)
         s3 = np.zeros((51, 47), np.float32)
-        self.lib.oc_check_prep(_ptr(crop), w, h, w, equalize, reflection, _ptr(u8), _ptr(s2), _ptr(s3))
-        return u8, s2, s3
+        f32 = np.zeros((51, 49), np.float32)
+        self.lib.oc_check_prep(_ptr(crop), w, h, w, equalize, reflection, _ptr(u8), _ptr(s2), _ptr(s3), _ptr(f32))
+        return u8, s2, s3, f32
 
     def sort_desc(self, keys):
         keys = np.ascontiguousarray(keys, np.float32)
@@ -167,13 +172,13 @@ class Oracle:
         return res
 
     def detect(self, img, params, trace_cap=0):
-        """Returns (faces [n,5] float32 x,y,w,h,score, trace dict or None)."""
+        """Returns (faces [n,8] float32 x,y,w,h,score,gender,smile,glasses, trace dict or None)."""
         img = np.ascontiguousarray(img, np.uint8)
         h, w = img.shape[:2]
         ch = 1 if img.ndim == 2 else img.shape[2]
         p = _oc_params(params)
         cap = 100000
-        raw = np.zeros(cap * 5, np.int32)
+        raw = np.zeros(cap * 8, np.int32)
         tr = None
         arrays = None
         if trace_cap > 0:
@@ -187,10 +192,11 @@ class Oracle:
                                w * ch, _ptr(raw), cap, ctypes.byref(tr) if tr is not None else None)
         if n < 0:
             return n, None
-        rec = raw[:5 * n].reshape(n, 5)
-        faces = np.zeros((n, 5), np.float32)
+        rec = raw[:8 * n].reshape(n, 8)
+        faces = np.zeros((n, 8), np.float32)
         faces[:, :4] = rec[:, :4]
         faces[:, 4] = rec[:, 4].view(np.float32)
+        faces[:, 5:] = rec[:, 5:]
         trace = None
         if tr is not None:
             nc = tr.n_cand
@@ -219,10 +225,11 @@ class Ref:
         L.ref_check_prep.argtypes = [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P]
         L.ref_equalize.argtypes = [_P, _I, _I, _I]
         L.ref_detect.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _I]
+        L.ref_detect_ex.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _I]
         L.ref_detect_loop.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _I]
         L.ref_detect_loop.restype = ctypes.c_long
         self.cnn = [L.ref_cnn_create(os.path.join(MODEL_DIR, MODEL_FILES[i]).encode(), INDEX_OUTPUT[i])
-                    for i in range(3)]
+                    for i in range(4)]
         if not all(self.cnn):
             raise RuntimeError("reference: model load failed")
 
@@ -277,18 +284,20 @@ class Ref:
     @staticmethod
     def _pack(p):
         ip = np.array([p["max_w"], p["max_h"], p["min_face"], p["max_face"], p["min_neighbors"], p["precision"],
-                       p["drop_detect"], p["equalize"], p["reflection"], p["detect_mode"]], np.int32)
+                       p["drop_detect"], p["equalize"], p["reflection"], p["detect_mode"],
+                       p.get("facial_analysis", 0)], np.int32)
         fp = np.array([p["scale_factor"], p["t1"], p["t2"], p["t3"]], np.float32)
         return ip, fp
 
     def detect(self, img, params):
+        """faces [n,8] float32: x, y, w, h, score, gender, smile, glasses."""
         img = np.ascontiguousarray(img, np.uint8)
         h, w = img.shape[:2]
         ch = 1 if img.ndim == 2 else img.shape[2]
         ip, fp = self._pack(params)
         cap = 100000
-        out = np.zeros((cap, 5), np.float32)
-        n = self.lib.ref_detect(_ptr(ip), _ptr(fp), _ptr(img), w, h, ch, w * ch, _ptr(out), cap)
+        out = np.zeros((cap, 8), np.float32)
+        n = self.lib.ref_detect_ex(_ptr(ip), _ptr(fp), _ptr(img), w, h, ch, w * ch, _ptr(out), cap)
         if n < 0:
             return n
         return out[:n].copy()

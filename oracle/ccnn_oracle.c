@@ -118,13 +118,13 @@ int oc_model_load(oc_model* m, const unsigned char* bytes, long n, int index_out
     m->ol_b = rd_arr(&r, m->ol);                             /* :381-385 */
     m->ol_tanh_w = rd_f32(&r);                               /* :387 */
     if (r.err) { oc_model_free(m); return -1; }
-    m->index_output = OC_MIN(index_output, m->ol - 1);       /* :389-390 */
+    m->index_output = OC_MIN(index_output, m->ol - 1);       /* :395-396 */
     if (m->index_output == 0) m->af_scale = -m->af_scale;
-    if (m->index_output < 0) { oc_model_free(m); return -1; }   /* multi-output maps: not on this path */
+    /* index_output < 0: every output map is computed (cnn_simd_cntk.cpp:935-949) */
     /* the layer wiring (add/add2 + two convs per sum) needs maps[l+1] == 2*maps[l] */
     if (m->maps[1] != 2 * m->maps[0] || m->maps[2] != 2 * m->maps[1]) { oc_model_free(m); return -1; }
     if (!((m->kw[0] == 4 && m->kh[0] == 4) || (m->kw[0] == 3 && m->kh[0] == 3))) { oc_model_free(m); return -1; }
-    if (m->hl / m->hl_scale + m->connect - 1 > m->maps[2]) { oc_model_free(m); return -1; }
+    if (m->hl / m->hl_scale + m->connect - 1 > m->maps[2] || m->connect > m->hl_nin) { oc_model_free(m); return -1; }
     return 0;
 }
 
@@ -163,9 +163,9 @@ static float lrelu_bn(float conv, float b, float w1, float w2, float bnb)
 }
 
 /* Valid cross-correlation at (x, y) of `src` (row stride sstep) with a kh x kw kernel.
- * conv_5x4 (cnnpp_cplusplus.cpp:158-187) accumulates one row sum at a time; every
- * other kernel (conv_3x3 :36-91, conv_4x4 :92-157, conv_8x7 :292-321) is a single
- * left-to-right chain over the rows. */
+ * conv_5x4 (cnnpp_cplusplus.cpp:158-187) accumulates one row sum at a time; conv_11x10
+ * (:352-380) is a chain that starts from 0.f; every other kernel (conv_3x3 :36-91,
+ * conv_4x4 :92-157, conv_8x7 :292-321) is a single left-to-right chain over the rows. */
 typedef float (*sample_fn)(const void* ctx, int x, int y);
 
 static float conv_at(const float* k, int kw, int kh, sample_fn f, const void* ctx, int x, int y)
@@ -177,6 +177,12 @@ static float conv_at(const float* k, int kw, int kh, sample_fn f, const void* ct
             for (int c = 1; c < 4; ++c) rs = rs + f(ctx, x + c, y + r) * k[r * 4 + c];
             d = (r == 0) ? rs : d + rs;
         }
+        return d;
+    }
+    if (kh == 11 && kw == 10) {
+        float d = 0.f;
+        for (int r = 0; r < 11; ++r)
+            for (int c = 0; c < 10; ++c) d = d + f(ctx, x + c, y + r) * k[r * 10 + c];
         return d;
     }
     float acc = f(ctx, x, y) * k[0];
@@ -271,6 +277,7 @@ int oc_cnn_forward(const oc_model* m, const float* img, int w, int h, int step, 
     /* HL: mulCN_add_tanhW (cnnpp_cplusplus.cpp:646-672; loop cnn_simd_cntk.cpp:874-883)
      * OL: mulC / mulC1_add / tanhW (cnnpp_cplusplus.cpp:952-984, 673-690; loop :885-900) */
     int S = m->hl_scale, G = m->hl / m->hl_scale, N = m->connect, io = m->index_output;
+    const int o0 = io < 0 ? 0 : io, o1 = io < 0 ? m->ol : io + 1;
     float* hv = (float*)malloc(sizeof(float) * m->hl);
     for (int y = 0; y < r3; ++y)
         for (int x = 0; x < c3; ++x) {
@@ -285,17 +292,20 @@ int oc_cnn_forward(const oc_model* m, const float* img, int w, int h, int step, 
                     c = (c + 1.f) * 0.5f;               /* 0.5f*c + 0.5f, as folded by the build */
                     hv[u] = m->hl_bn_w[t] * c + m->hl_bn_b[t];
                 }
-            float acc = 0.f;
-            for (int g = 0; g < G; ++g)
-                for (int t = 0; t < S; ++t) {
-                    float wv = m->ol_w[(long)io * m->hl + t * G + g];
-                    if (g == 0 && t == 0) acc = hv[0] * wv;
-                    else acc = hv[S * g + t] * wv + acc;
-                }
-            float c = acc + m->ol_b[io];
-            c *= m->ol_tanh_w;
-            c = rtanh(c);
-            out[o] = (m->af_scale == 0.f) ? (c + 1.f) * 0.5f : m->af_scale * c;
+            for (int oi = o0; oi < o1; ++oi) {
+                float acc = 0.f;
+                for (int g = 0; g < G; ++g)
+                    for (int t = 0; t < S; ++t) {
+                        float wv = m->ol_w[(long)oi * m->hl + t * G + g];
+                        if (g == 0 && t == 0) acc = hv[0] * wv;
+                        else acc = hv[S * g + t] * wv + acc;
+                    }
+                float c = acc + m->ol_b[oi];
+                c *= m->ol_tanh_w;
+                c = rtanh(c);
+                /* 0.5f*c + 0.5f folded as in the hidden layer when af_scale is 0 */
+                out[(long)(oi - o0) * c3 * r3 + o] = (m->af_scale == 0.f) ? (c + 1.f) * 0.5f : m->af_scale * c;
+            }
         }
     free(hv);
     for (int k = 0; k < M1; ++k) free(P1[k]);
@@ -455,10 +465,11 @@ void oc_equalize(unsigned char* img, int w, int h, int step)
  * (check resizer, :829), FloatToUChar (image_proc.cpp:357-368), equalizeImage, then
  * UCharToFloat with offset 2 (:416-427) and UCharToFloat_inv (:478-490) or offset 0. */
 void oc_check_prep(const float* crop, int cw, int ch, int cstep, int equalize, int reflection,
-                   unsigned char* u8_out, float* s2_out, float* s3_out)
+                   unsigned char* u8_out, float* s2_out, float* s3_out, float* f32_out)
 {
     float r32[49 * 51];
     oc_resize_f32(crop, cw, ch, cstep, r32, 49, 51, 49, 0);
+    if (f32_out) memcpy(f32_out, r32, sizeof(r32));
     for (int i = 0; i < 49 * 51; ++i) u8_out[i] = (unsigned char)(r32[i] + 0.5f);
     if (equalize) oc_equalize(u8_out, 49, 51, 49);
     for (int y = 0; y < 51; ++y)
@@ -597,7 +608,9 @@ void oc_sort_desc(int* perm, const float* key, int n)
 /* ------------------------------------------------------------------------------ */
 
 typedef struct { int x, y, w, h, x2, y2; } rect_t;            /* type.h:107-137 */
-typedef struct { rect_t r; float score, scale; int knn, num_detect, checked; } det_t;
+/* Detection (cnn_detector_v3.h:185-216); fd holds the facial_data entries as
+ * {gender, smile, glasses} triples (nfd of them, heap-owned by the detection). */
+typedef struct { rect_t r; float score, scale; int knn, num_detect, checked; int nfd; int* fd; } det_t;
 typedef struct { det_t* v; int n, cap; } dvec_t;
 
 static rect_t mk_rect(int x, int y, int w, int h)
@@ -611,8 +624,26 @@ static void dpush(dvec_t* d, rect_t r, float score, float scale, int knn)
         d->cap = d->cap ? 2 * d->cap : 64;
         d->v = (det_t*)realloc(d->v, sizeof(det_t) * (size_t)d->cap);
     }
-    det_t e = { r, score, scale, knn, 0, 0 };
+    det_t e = { r, score, scale, knn, 0, 0, 0, NULL };
     d->v[d->n++] = e;
+}
+static int* fd_dup(const int* fd, int n)
+{
+    if (n == 0) return NULL;
+    int* a = (int*)malloc(sizeof(int) * 3 * (size_t)n);
+    memcpy(a, fd, sizeof(int) * 3 * (size_t)n);
+    return a;
+}
+static void fd_append(det_t* d, const int* fd, int n)
+{
+    if (n == 0) return;
+    d->fd = (int*)realloc(d->fd, sizeof(int) * 3 * (size_t)(d->nfd + n));
+    memcpy(d->fd + 3 * d->nfd, fd, sizeof(int) * 3 * (size_t)n);
+    d->nfd += n;
+}
+static void dvec_free_fd(dvec_t* d)
+{
+    for (int i = 0; i < d->n; ++i) { free(d->v[i].fd); d->v[i].fd = NULL; d->v[i].nfd = 0; }
 }
 /* Rect::overlap (type.h:154-173): intersection over the bounding box of both */
 static float rect_overlap(const rect_t* a, const rect_t* b)
@@ -669,7 +700,7 @@ static eff_t effective(const oc_params* p)
     case 4: e.type_check = 2; e.min_num_detect = 2; break;
     default: break;
     }
-    if (e.detect_mode == 0) e.drop_detect = 0;      /* :961-964 */
+    if (e.detect_mode == 0 || p->facial_analysis) e.drop_detect = 0;      /* :961-964 */
     return e;
 }
 
@@ -729,7 +760,7 @@ int oc_plan(const oc_params* p, int pattern_h, int w, int h, float* scales, int*
 }
 
 typedef struct {
-    const oc_model *m1, *m2, *m3;
+    const oc_model *m1, *m2, *m3, *mfa;
     const oc_params* p;
     eff_t e;
     const float* gray; int gw, gh, gstep;
@@ -740,7 +771,7 @@ typedef struct {
 /* The stage-2/3 evaluation of one crop (cnn_detector_v3.cpp:1485-1818, from the crop
  * geometry onwards). Returns flag_object; fills k1, m1, k2, m2. */
 static int check_eval(const dctx_t* c, float scale, int px, int py, int mod,
-                      int* k1o, float* m1o, int* k2o, float* m2o)
+                      int* k1o, float* m1o, int* k2o, float* m2o, float* crop49, int* geo)
 {
     const float inv = 1.f / scale;
     int x = (int)((float)(px - 6) * inv);          /* (ext 11 + x 2) / 2 = 6 */
@@ -763,7 +794,8 @@ static int check_eval(const dctx_t* c, float scale, int px, int py, int mod,
     unsigned char u8[49 * 51];
     float in2[47 * 51], in3[47 * 51], out[64];
     oc_check_prep(c->gray + (long)ry * c->gstep + rx, rcols, rrows, c->gstep,
-                  c->e.equalize, c->e.reflection, u8, in2, in3);
+                  c->e.equalize, c->e.reflection, u8, in2, in3, crop49);
+    if (geo) { geo[0] = rx; geo[1] = ry; geo[2] = rcols; geo[3] = rrows; }
     const int mn = (c->e.adapt_min_neighbors && scale > 0.7f && c->e.min_neighbors > 1)
                    ? c->e.min_neighbors - 1 : c->e.min_neighbors;
     float max1 = -10.f, max2 = -10.f;
@@ -798,6 +830,28 @@ static float check_score(float score0, int k1, float m1, int k2, float m2)
     return score0 + (float)k1 * OC_MAX(-1.7159f, m1) + (float)k2 * OC_MAX(-1.7159f, m2);
 }
 
+/* Facial analysis of an accepted window (cnn_detector_v3.cpp:1731-1756): the landmarks
+ * network on the 49x51 FP32 crop; five (x, y) landmarks scaled by the crop size, the
+ * window rejected when all five fall in the crop's top-left quadrant, and gender /
+ * smile / glasses rounded from outputs 10-12. Returns 0 when rejected, else 1 with
+ * fd = {gender, smile, glasses}. */
+static int facial_analysis(const oc_model* mfa, const float* crop49, int rcols, int rrows, int* fd)
+{
+    float o[16];
+    if (mfa->ol > 16 || oc_cnn_forward(mfa, crop49, 49, 51, 49, o) < 0) return 1;
+    int zero = 0;
+    for (int t = 0; t < 5; ++t) {
+        const int lx = (int)(o[2 * t] * (float)rcols);
+        const int ly = (int)(o[2 * t + 1] * (float)rrows);
+        if (lx < (rcols >> 1) && ly < (rrows >> 1)) zero++;
+    }
+    if (zero == 5) return 0;
+    fd[0] = (int)(o[10] + 0.5f);
+    fd[1] = (int)(o[11] + 0.5f);
+    fd[2] = (int)(o[12] + 0.5f);
+    return 1;
+}
+
 /* CPUCheckDetect with the drop check (cnn_detector_v3.cpp:1485-1818). */
 static void cpu_check_detect(dctx_t* c, int rect_size, int px, int py, float score0, float scale,
                              int mod, int cand)
@@ -815,14 +869,24 @@ static void cpu_check_detect(dctx_t* c, int rect_size, int px, int py, float sco
         }
     }
     int k1, k2; float m1, m2;
-    int flag = check_eval(c, scale, px, py, mod, &k1, &m1, &k2, &m2);
+    float crop49[49 * 51];
+    int geo[4];
+    int flag = check_eval(c, scale, px, py, mod, &k1, &m1, &k2, &m2, crop49, geo);
     if (cand >= 0) {
         c->tr->c_mod[cand] = mod; c->tr->c_k1[cand] = k1; c->tr->c_m1[cand] = m1;
         c->tr->c_k2[cand] = k2; c->tr->c_m2[cand] = m2;
         c->tr->c_status[cand] = flag ? 2 : 3;
     }
     if (flag) {
+        int fd[3];
+        if (c->mfa) {
+            if (!facial_analysis(c->mfa, crop49, geo[2], geo[3], fd)) {
+                if (cand >= 0) c->tr->c_status[cand] = 4;
+                return;                                /* :1749: no retry */
+            }
+        }
         dpush(&c->rects, nr, check_score(score0, k1, m1, k2, m2), scale, OC_MIN(k1, k2));
+        if (c->mfa) fd_append(&c->rects.v[c->rects.n - 1], fd, 1);   /* :1762-1765 */
     } else if (c->e.double_check && mod == 0) {
         cpu_check_detect(c, rect_size, px, py, score0, scale, 1, cand);
     }
@@ -900,6 +964,8 @@ static void merger(dvec_t* out, dvec_t* in, float threshold, int del)
     for (int a = 0; a < in->n; ++a) {
         if (in->v[a].checked) continue;
         det_t mx = in->v[a], mn = in->v[a];
+        mx.fd = fd_dup(in->v[a].fd, in->v[a].nfd);             /* the copy owns its facial_data */
+        mn.fd = NULL; mn.nfd = 0;
         for (int b = a + 1; b < in->n; ++b) {
             det_t* it2 = &in->v[b];
             if (it2->checked) continue;
@@ -936,11 +1002,12 @@ static void merger(dvec_t* out, dvec_t* in, float threshold, int del)
                 mx.score = OC_MAX(mx.score, it2->score);
                 mx.knn = OC_MAX(mx.knn, it2->knn);
                 mx.scale = OC_MIN(mx.scale, it2->scale);
+                fd_append(&mx, it2->fd, it2->nfd);             /* :2911-2914 */
                 mx.num_detect++;
                 it2->checked = 1;
             }
         }
-        if (del && mn.r.x2 == 0) continue;
+        if (del && mn.r.x2 == 0) { free(mx.fd); continue; }
         mx.r.x = (mn.r.x + mx.r.x) >> 1;
         mx.r.y = (mn.r.y + mx.r.y) >> 1;
         mx.r.x2 = (mn.r.x2 + mx.r.x2) >> 1;
@@ -966,8 +1033,8 @@ int oc_check_candidate(const oc_model* m2, const oc_model* m3, const oc_params* 
     c.gray = gray; c.gw = gw; c.gh = gh; c.gstep = gstep;
     int k1, k2; float f1, f2;
     int mod = 0;
-    int flag = check_eval(&c, scale, px, py, 0, &k1, &f1, &k2, &f2);
-    if (!flag && c.e.double_check) { mod = 1; flag = check_eval(&c, scale, px, py, 1, &k1, &f1, &k2, &f2); }
+    int flag = check_eval(&c, scale, px, py, 0, &k1, &f1, &k2, &f2, NULL, NULL);
+    if (!flag && c.e.double_check) { mod = 1; flag = check_eval(&c, scale, px, py, 1, &k1, &f1, &k2, &f2, NULL, NULL); }
     res[0] = (float)flag; res[1] = (float)mod; res[2] = (float)k1; res[3] = (float)k2;
     res[4] = f1; res[5] = f2;
     const float inv = 1.f / scale;
@@ -976,13 +1043,27 @@ int oc_check_candidate(const oc_model* m2, const oc_model* m3, const oc_params* 
     return 0;
 }
 
-int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_params* p,
-              const unsigned char* img, int w, int h, int ch, int step,
+/* Median of one facial_data key as FaceDetector::Detect reports it
+ * (CompactCNNLibAPI_v2.cpp:296-319): sorted descending, the element at size/2. */
+static int fd_median(const det_t* d, int key)
+{
+    int* v = (int*)malloc(sizeof(int) * (size_t)d->nfd);
+    for (int i = 0; i < d->nfd; ++i) v[i] = d->fd[3 * i + key];
+    for (int i = 1; i < d->nfd; ++i)
+        for (int j = i; j > 0 && v[j - 1] < v[j]; --j) { int t = v[j]; v[j] = v[j - 1]; v[j - 1] = t; }
+    int r = v[d->nfd / 2];
+    free(v);
+    return r;
+}
+
+int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
+              const oc_params* p, const unsigned char* img, int w, int h, int ch, int step,
               oc_face* out, int cap, oc_trace* trace)
 {
     dctx_t c;
     memset(&c, 0, sizeof(c));
     c.m1 = m1; c.m2 = m2; c.m3 = m3; c.p = p; c.e = effective(p); c.tr = trace;
+    c.mfa = p->facial_analysis ? mfa : NULL;
     if (trace) { trace->n_cand = 0; trace->n_levels = 0; trace->n_scales = 0; trace->n_raw = 0; }
     if (ch != 1 && ch != 3 && ch != 4) return -1;
     /* ext_pattern_size_cd (cnn_detector_v3.cpp:762-764, 879-882) */
@@ -1062,7 +1143,10 @@ int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const 
             merger(&tmp, &raw, 0.5f, 0);
             if (c.e.min_num_detect > 1) {
                 int k = 0;
-                for (int i = 0; i < tmp.n; ++i) if (!(tmp.v[i].num_detect < c.e.min_num_detect)) tmp.v[k++] = tmp.v[i];
+                for (int i = 0; i < tmp.n; ++i) {
+                    if (!(tmp.v[i].num_detect < c.e.min_num_detect)) tmp.v[k++] = tmp.v[i];
+                    else free(tmp.v[i].fd);
+                }
                 tmp.n = k;
             }
             if (tmp.n > 0) merger(&fin, &tmp, 0.2f, 1);
@@ -1077,10 +1161,18 @@ int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const 
             if (nout < cap) {
                 out[nout].x = r.x; out[nout].y = r.y; out[nout].w = r.w; out[nout].h = r.h;
                 out[nout].score = res->v[i].score;
+                out[nout].gender = out[nout].smile = out[nout].glasses = -1;
+                if (res->v[i].nfd > 0) {
+                    out[nout].gender = fd_median(&res->v[i], 0);
+                    out[nout].smile = fd_median(&res->v[i], 1);
+                    out[nout].glasses = fd_median(&res->v[i], 2);
+                }
             }
             nout++;
         }
     }
+    /* raw holds shallow copies of c.rects; tmp and fin own their facial_data */
+    dvec_free_fd(&c.rects); dvec_free_fd(&tmp); dvec_free_fd(&fin);
     free(raw.v); free(tmp.v); free(fin.v); free(c.rects.v);
     free(gray); free(pre_img);
     return nout;

@@ -53,7 +53,8 @@ typedef struct {
 int  oc_model_load(oc_model* m, const unsigned char* bytes, long n, int index_output);
 void oc_model_free(oc_model* m);
 void oc_output_size(const oc_model* m, int w, int h, int* ow, int* oh);
-/* Dense output [oh][ow]; returns 0, or -1 if the image is below the model's minimum. */
+/* Dense output [oh][ow] ([ol][oh][ow] when index_output < 0: every output map); returns 0,
+ * or -1 if the image is below the model's minimum. */
 int  oc_cnn_forward(const oc_model* m, const float* img, int w, int h, int step, float* out);
 
 /* Image ops */
@@ -63,8 +64,11 @@ void oc_resize_f32(const float* src, int sw, int sh, int sstep,
 void oc_resize_u8(const unsigned char* src, int sw, int sh, int sstep,
                   unsigned char* dst, int dw, int dh, int dstep, int type);
 void oc_equalize(unsigned char* img, int w, int h, int step);
+/* The check crop pipeline of CPUCheckDetect (cnn_detector_v3.cpp:1565-1575, 1617, 1638):
+ * nearest-neighbour resize to 49x51 (f32_out, may be NULL), +0.5-truncate to u8,
+ * optional equalize, stage-2 input (cols 2..48) and stage-3 input (mirrored or plain). */
 void oc_check_prep(const float* crop, int cw, int ch, int cstep, int equalize, int reflection,
-                   unsigned char* u8_out, float* s2_out, float* s3_out);
+                   unsigned char* u8_out, float* s2_out, float* s3_out, float* f32_out);
 
 /* libstdc++ std::sort restated over an index permutation keyed by float, with the
  * detector's comparator comp(a, b) = key[b] < key[a]. perm is sorted in place. */
@@ -81,9 +85,11 @@ typedef struct {
     int detect_mode;    /* 0 disable, 1 sync, 2 async */
     float scale_factor;
     float t1, t2, t3;
+    int facial_analysis;
 } oc_params;
 
-typedef struct { int x, y, w, h; float score; } oc_face;
+/* FaceDetector::Face (CompactCNNLib.h:73-86); gender/smile/glasses -1 without facial data */
+typedef struct { int x, y, w, h; float score; int gender, smile, glasses; } oc_face;
 
 /* Per-level and per-candidate record of one detection, for stage-by-stage parity. */
 typedef struct {
@@ -94,7 +100,8 @@ typedef struct {
     /* candidates: every response > t1, level compute order, raster order */
     int cap, n_cand;
     int* c_scl; int* c_x; int* c_y; float* c_score;
-    int* c_status;               /* 0 scan-drop, 1 check-drop, 2 accepted, 3 rejected */
+    int* c_status;               /* 0 scan-drop, 1 check-drop, 2 accepted, 3 rejected,
+                                    4 accepted by the checks, rejected by facial analysis */
     int* c_mod;                  /* mod of the final evaluation (0 or 1), -1 if none */
     int* c_k1; float* c_m1; int* c_k2; float* c_m2;   /* of the final evaluation */
     int n_raw;                   /* detections before Merger */
@@ -102,8 +109,8 @@ typedef struct {
 
 /* Full detection of one frame with the reference's semantics.
  * Returns the face count (<= cap written), 0 for images below 47x51, -1 on error. */
-int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_params* p,
-              const unsigned char* img, int w, int h, int ch, int step,
+int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
+              const oc_params* p, const unsigned char* img, int w, int h, int ch, int step,
               oc_face* out, int cap, oc_trace* trace);
 
 /* One candidate's check (CPUCheckDetect without the drop logic): results of mod 0 and,

@@ -20,6 +20,7 @@
 #include "image_resize.h"
 #include "cnn_simd_cntk.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -141,7 +142,7 @@ int ref_equalize(unsigned char* img, int w, int h, int step)
 // iparams: [max_w, max_h, min_face, max_face, min_neighbors, precision,
 //           drop_detect, equalize, reflection, detect_mode]
 // fparams: [scale_factor, t1, t2, t3]
-static CNNDetector* make_detector(const int* iparams, const float* fparams, int channels)
+static CNNDetector* make_detector(const int* iparams, const float* fparams, int channels, bool facial = false)
 {
     CNNDetector::Param cp;
     CNNDetector::AdvancedParam ap;
@@ -163,7 +164,7 @@ static CNNDetector* make_detector(const int* iparams, const float* fparams, int 
     for (int i = 0; i < 3; ++i) ap.path_model[i] = "";
     ap.index_output[0] = 1; ap.index_output[1] = 0; ap.index_output[2] = 0;
     ap.device_info = false;
-    ap.facial_analysis = false;
+    ap.facial_analysis = facial;
     ap.double_check = true;
     ap.adapt_min_neighbors = false;
     ap.gray_image_only = channels == 1;
@@ -197,6 +198,49 @@ int ref_detect(const int* iparams, const float* fparams,
         out[5 * i + 2] = (float)dets[i].rect.width;
         out[5 * i + 3] = (float)dets[i].rect.height;
         out[5 * i + 4] = dets[i].score;
+    }
+    return n;
+}
+
+// As ref_detect with facial analysis selectable (iparams[10]); out: 8 values per face
+// (x, y, w, h, score, gender, smile, glasses). gender / smile / glasses are reported the
+// way FaceDetector::Detect does (CompactCNNLibAPI_v2.cpp:296-319): the facial_data
+// entries of the detection sorted by the key in descending order, the element at
+// size/2; -1 when there are none.
+int ref_detect_ex(const int* iparams, const float* fparams,
+                  const unsigned char* img, int w, int h, int ch, int step,
+                  float* out, int cap)
+{
+    CNNDetector* d = make_detector(iparams, fparams, ch, iparams[10] != 0);
+    if (!d) return -1;
+    std::vector<CNNDetector::Detection> dets;
+    SIMD::Image_8u image(w, h, ch, const_cast<unsigned char*>(img), step);
+    int rc = d->Detect(dets, image);
+    delete d;
+    if (rc < 0) return rc;
+    int n = (int)dets.size();
+    int m = n < cap ? n : cap;
+    for (int i = 0; i < m; ++i)
+    {
+        float* o = out + 8 * i;
+        o[0] = (float)dets[i].rect.x;
+        o[1] = (float)dets[i].rect.y;
+        o[2] = (float)dets[i].rect.width;
+        o[3] = (float)dets[i].rect.height;
+        o[4] = dets[i].score;
+        o[5] = o[6] = o[7] = -1.f;
+        std::vector<CNNDetector::FacialData> fd = dets[i].facial_data;
+        const size_t k = fd.size();
+        if (k == 0) continue;
+        std::sort(fd.begin(), fd.end(), [](const CNNDetector::FacialData& a, const CNNDetector::FacialData& b)
+                  { return a.gender > b.gender; });
+        o[5] = (float)fd[k / 2].gender;
+        std::sort(fd.begin(), fd.end(), [](const CNNDetector::FacialData& a, const CNNDetector::FacialData& b)
+                  { return a.smile > b.smile; });
+        o[6] = (float)fd[k / 2].smile;
+        std::sort(fd.begin(), fd.end(), [](const CNNDetector::FacialData& a, const CNNDetector::FacialData& b)
+                  { return a.glasses > b.glasses; });
+        o[7] = (float)fd[k / 2].glasses;
     }
     return n;
 }

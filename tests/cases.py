@@ -61,6 +61,13 @@ CASES = {
     "photo3_min40_max200": (photo3, dict(min_face=40, max_face=200)),
     "fhd_w0_min40_max240": (lambda: fhd(0), dict(min_face=40, max_face=240)),
     "fhd_w1_min40_max240": (lambda: fhd(1), dict(min_face=40, max_face=240)),
+    # facial analysis: the landmarks network on accepted windows, which may drop them,
+    # and the gender / smile / glasses attributes of each face
+    "vga_s3_min40_facial": (lambda: vga(3), dict(min_face=40, facial_analysis=1)),
+    "vga_s11_min20_facial": (lambda: vga(11, smin=20), dict(min_face=20, facial_analysis=1)),
+    "photo3_min20_facial": (photo3, dict(min_face=20, facial_analysis=1)),
+    "photo3_min40_max200_facial_high": (photo3, dict(min_face=40, max_face=200, precision=3, facial_analysis=1)),
+    "vga_s11_gray_min24_facial": (lambda: _gray(vga(11, smin=20)), dict(min_face=24, facial_analysis=1)),
 }
 
 # 1-channel frames with min face >= 34 take the gray_image_only pre-resize path
@@ -87,7 +94,8 @@ def api_param(frame, overrides):
                  max_face_height=p["max_face"], min_neighbors=p["min_neighbors"],
                  detect_precision=p["precision"], drop_detect=bool(p["drop_detect"]),
                  equalize=bool(p["equalize"]), reflection=bool(p["reflection"]), detect_mode=p["detect_mode"],
-                 scale_factor=p["scale_factor"], treshold=[p["t1"], p["t2"], p["t3"]])
+                 scale_factor=p["scale_factor"], treshold=[p["t1"], p["t2"], p["t3"]],
+                 facial_analysis=bool(p["facial_analysis"]))
 
 
 def frame_digest(frame):

@@ -24,7 +24,8 @@ from oracle.bind import Oracle, Ref  # noqa: E402
 
 
 def faces_list(a):
-    return [[int(v[0]), int(v[1]), int(v[2]), int(v[3]), float(v[4])] for v in a]
+    """x, y, w, h, score, gender, smile, glasses"""
+    return [[int(v[0]), int(v[1]), int(v[2]), int(v[3]), float(v[4]), int(v[5]), int(v[6]), int(v[7])] for v in a]
 
 
 def main():
@@ -77,6 +78,11 @@ def main():
     crop = (rng.random((44, 39)) * 255).astype(np.float32)
     u8, s2, s3 = ref.check_prep(crop, 1, 1)
     vec["prep_crop"], vec["prep_u8"], vec["prep_s2"], vec["prep_s3"] = crop, u8, s2, s3
+    # the landmarks network (all 13 outputs) on two 49x51 crops
+    fa_in = np.stack([(rng.random((51, 49)) * 255).astype(np.float32),
+                      np.round(rng.random((51, 49)) * 255).astype(np.float32)])
+    vec["fa_in"] = fa_in
+    vec["fa_out"] = np.stack([ref.forward(3, x)[:, 0] for x in fa_in])
     np.savez_compressed(os.path.join(HERE, "vectors.npz"), **vec)
     print("wrote", os.path.join(HERE, "detections.json"), os.path.join(HERE, "vectors.npz"))
 

@@ -29,16 +29,28 @@ def golden():
         return json.load(f)["cases"]
 
 
+@pytest.fixture(scope="module")
+def vectors():
+    return dict(np.load(os.path.join(GOLDEN, "vectors.npz")))
+
+
 def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
 def test_gray_bitexact(api, oracle):
     rng = np.random.default_rng(1)
-    for shape in [(61, 77, 3), (33, 45, 4), (29, 31), (1080, 1920, 3)]:
+    # 4-pixel-vector path (4-byte aligned rows; 16-byte for BGRA) with full and partial
+    # last quads, and the byte path for other row steps
+    for shape in [(61, 77, 3), (33, 45, 4), (29, 31), (1080, 1920, 3), (37, 84, 3), (37, 83, 3), (21, 80, 4),
+                  (21, 81, 4), (19, 64, 1), (19, 66, 1), (5, 3, 3)]:
         img = rng.integers(0, 256, size=shape, dtype=np.uint8)
         print("gray", shape, flush=True)
         assert np.array_equal(bits(api.stage_gray(img)), bits(oracle.gray(img)))
+    # a padded row step (a view into a wider frame)
+    wide = rng.integers(0, 256, size=(40, 100, 3), dtype=np.uint8)
+    for sub in [wide[:, :70], wide[3:, 5:90]]:
+        assert np.array_equal(bits(api.stage_gray(sub)), bits(oracle.gray(np.ascontiguousarray(sub))))
 
 
 def test_gray_all_bgr_triples(api, oracle):
@@ -124,6 +136,8 @@ def test_detect_matches_reference(api, golden, name):
     got = [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces]
     assert got == [e[:4] for e in exp]
     assert [int(v) for v in bits(faces["score"])] == g["score_bits"]
+    # facial attributes (-1 without facial analysis)
+    assert [[int(r["gender"]), int(r["smile"]), int(r["glasses"])] for r in faces] == [e[5:8] for e in exp]
 
 
 def test_trace_matches_oracle(api, oracle):
@@ -154,6 +168,36 @@ def test_trace_matches_oracle(api, oracle):
     assert bits(res[:, 4]).tolist() == bits(tr["c_m1"][ev]).tolist()
 
 
+def test_landmarks_network_bitexact(api, oracle, vectors):
+    rng = np.random.default_rng(12)
+    inputs = list(vectors["fa_in"]) + [(rng.random((51, 49)) * 255).astype(np.float32) for _ in range(4)]
+    for x in inputs:
+        a = api.stage_landmarks(x)
+        b = oracle.forward(3, x)[:, 0, 0]
+        assert np.array_equal(bits(a), bits(b))
+    for x, y in zip(vectors["fa_in"], vectors["fa_out"]):
+        assert np.array_equal(bits(api.stage_landmarks(x)), bits(y))
+
+
+def test_facial_analysis_per_candidate_matches_oracle(api, oracle):
+    # every window the checks accept: the landmarks' verdict and the attributes
+    from cases import photo3
+    f = photo3()
+    over = dict(min_face=20, facial_analysis=1)
+    p = params_for(f, over)
+    _, tr = oracle.detect(f, p, trace_cap=200000)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    d.Detect(f)
+    t = d.last_trace()
+    fa = d.last_facial()
+    assert len(fa) == len(tr["c_x"])
+    ev = np.isin(tr["c_status"], [2, 4])          # accepted by the checks
+    assert (t["res"][ev, 0] == 1).all()
+    assert (fa[ev, 0] == (tr["c_status"][ev] == 2)).all()
+    assert ev.sum() > 5 and (tr["c_status"] == 4).sum() > 0
+
+
 def test_repeated_detect_is_stable(api):
     from cases import vga
     f = vga(0)
@@ -163,6 +207,20 @@ def test_repeated_detect_is_stable(api):
     for _ in range(3):
         again = d.Detect(f)[1]
         assert np.array_equal(first, again)
+
+
+def test_detect_on_row_strided_view(api):
+    # ImageData with a row step wider than cols * channels (a crop of a bigger frame)
+    from cases import vga
+    big = np.zeros((520, 700, 3), np.uint8)
+    big[17:497, 31:671] = vga(3)
+    view = big[17:497, 31:671]
+    assert view.strides[0] == 2100
+    d = api.FaceDetector()
+    assert d.Init(api_param(vga(3), dict(min_face=40))) == 0
+    a = d.Detect(view)
+    b = d.Detect(np.ascontiguousarray(view))
+    assert a[0] == b[0] and a[0] > 0 and np.array_equal(a[1], b[1])
 
 
 def test_batch_matches_single_frame_results(api, golden):

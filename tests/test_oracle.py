@@ -50,6 +50,8 @@ def test_oracle_detections_match_golden(oracle, golden, name):
     faces, _ = oracle.detect(f, params_for(f, over))
     assert faces[:, :4].astype(int).tolist() == [e[:4] for e in g["faces"]]
     assert [int(v) for v in bits(faces[:, 4])] == g["score_bits"]
+    # gender / smile / glasses (-1 without facial analysis)
+    assert faces[:, 5:8].astype(int).tolist() == [e[5:8] for e in g["faces"]]
 
 
 def test_oracle_small_vectors(oracle, vectors):
@@ -63,9 +65,11 @@ def test_oracle_small_vectors(oracle, vectors):
     assert np.array_equal(bits(oracle.forward(0, v["s1_in"])), bits(v["s1_out"]))
     assert np.array_equal(bits(oracle.forward(1, v["s2_in"])), bits(v["s2_out"]))
     assert np.array_equal(bits(oracle.forward(2, v["s2_in"])), bits(v["s3_out"]))
-    u8, s2, s3 = oracle.check_prep(v["prep_crop"], 1, 1)
+    u8, s2, s3, _ = oracle.check_prep(v["prep_crop"], 1, 1)
     assert np.array_equal(u8, v["prep_u8"])
     assert np.array_equal(s2, v["prep_s2"]) and np.array_equal(s3, v["prep_s3"])
+    for x, y in zip(v["fa_in"], v["fa_out"]):
+        assert np.array_equal(bits(oracle.forward(3, x)[:, 0, 0]), bits(y))
 
 
 @pytest.mark.skipif(not os.path.exists(REF_O0), reason="compiled reference not built (make -C oracle ref)")
@@ -195,26 +199,32 @@ def _net64(m, x):
     s2 = [sum(g) for g in sums(p2)]
     a3 = [act(conv(s2[i // 2], K[2][i]), 2, i) for i in range(M[2])]
     S, G, N = m.hl_scale, m.hl // m.hl_scale, m.connect
-    hw = arr(m.hl_w, m.hl * N).reshape(m.hl, N)
+    # weights per hidden unit: `connect` of them, or one per L3 map when fully connected
+    hw = arr(m.hl_w, m.hl * m.hl_nin).reshape(m.hl, m.hl_nin)[:, :N]
     hb, ht = arr(m.hl_b, m.hl), arr(m.hl_tanh_w, G)
     bw, bb = arr(m.hl_bn_w, S), arr(m.hl_bn_b, S)
-    ow = arr(m.ol_w, m.ol * m.hl).reshape(m.ol, m.hl)[m.index_output]
-    ob = arr(m.ol_b, m.ol)[m.index_output]
+    outs = range(m.ol) if m.index_output < 0 else [m.index_output]
 
     def rt(c):
         return np.sign(c) * (1 - 1 / (1 + np.abs(c) + c * c + 1.41645 * c ** 4))
 
-    o = 0
+    h = []
     for g in range(G):
         for t in range(S):
             u = S * g + t
             c = hb[u] + sum(a3[g + j] * hw[u, j] for j in range(N))
-            h = bw[t] * (0.5 * rt(c * ht[g]) + 0.5) + bb[t]
-            o = o + h * ow[t * G + g]
-    return m.af_scale * rt((o + ob) * m.ol_tanh_w)
+            h.append(bw[t] * (0.5 * rt(c * ht[g]) + 0.5) + bb[t])
+    res = []
+    for k in outs:
+        ow = arr(m.ol_w, m.ol * m.hl).reshape(m.ol, m.hl)[k]
+        ob = arr(m.ol_b, m.ol)[k]
+        o = sum(h[S * g + t] * ow[t * G + g] for g in range(G) for t in range(S))
+        v = rt((o + ob) * m.ol_tanh_w)
+        res.append(0.5 * v + 0.5 if m.af_scale == 0 else m.af_scale * v)
+    return res[0] if m.index_output >= 0 else np.stack(res)
 
 
-@pytest.mark.parametrize("stage,shape", [(0, (51, 47)), (0, (90, 75)), (1, (51, 47)), (2, (51, 47))])
+@pytest.mark.parametrize("stage,shape", [(0, (51, 47)), (0, (90, 75)), (1, (51, 47)), (2, (51, 47)), (3, (51, 49))])
 def test_networks_match_float64_restatement(oracle, stage, shape):
     rng = np.random.default_rng(10 + stage)
     x = (rng.random(shape) * 255).round().astype(np.float32)
@@ -222,6 +232,9 @@ def test_networks_match_float64_restatement(oracle, stage, shape):
     exp = _net64(oracle.models[stage], x)
     assert got.shape == exp.shape
     assert np.max(np.abs(got - exp)) < 2e-5 * 1.7159
+    if stage == 3:
+        # the landmarks network: 13 output maps of one value each
+        assert got.shape == (13, 1, 1)
 
 
 def test_stage1_output_size_rule(oracle):
@@ -267,11 +280,11 @@ def test_equalize_closed_form_and_reference_property(oracle):
 def test_check_prep_closed_form(oracle):
     rng = np.random.default_rng(8)
     crop = (rng.random((49 * 2, 51)) * 255).astype(np.float32)[:51, :49]
-    u8, s2, s3 = oracle.check_prep(crop, equalize=0, reflection=1)
+    u8, s2, s3, f32 = oracle.check_prep(crop, equalize=0, reflection=1)
     assert np.array_equal(u8, (np.ascontiguousarray(crop) + np.float32(0.5)).astype(np.uint8))   # 49x51: a copy
     assert np.array_equal(s2, u8[:, 2:49].astype(np.float32))
     assert np.array_equal(s3, u8[:, 46::-1].astype(np.float32))
-    _, _, s3b = oracle.check_prep(crop, equalize=0, reflection=0)
+    _, _, s3b, _ = oracle.check_prep(crop, equalize=0, reflection=0)
     assert np.array_equal(s3b, u8[:, :47].astype(np.float32))
 
 
