@@ -188,6 +188,17 @@ constexpr int P2W = TX + 3, P2H = TY + 4;             // pooled L2 / second sums
 constexpr int P1N = P1W * P1H, P2N = P2W * P2H;
 }  // namespace s1
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): workgroup b
+// runs on XCD b % 8. Renumber them so that each XCD takes one contiguous run of tiles,
+// whose input windows overlap and share that XCD's L2. A bijection on [0, n).
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    constexpr int NX = 8;
+    const int xcd = b % NX, local = b / NX;
+    const int q = n / NX, r = n % NX;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 __device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n, int t, bool chunks)
 {
     int lo = 0, hi = n - 1;
@@ -209,9 +220,10 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
     __shared__ float s_p2[N::M2 * P2N];
 
     const int tid = threadIdx.x;
-    const int li = find_level(levels, nlevels, blockIdx.x, false);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int li = find_level(levels, nlevels, tile, false);
     const LevelDesc L = levels[li];
-    const int t = blockIdx.x - L.tile_off;
+    const int t = tile - L.tile_off;
     const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
     const int x0 = 4 * ox0, y0 = 4 * oy0;
 
