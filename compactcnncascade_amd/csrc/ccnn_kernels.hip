@@ -13,9 +13,9 @@
 //   k_count / k_scan / k_scatter
 //                  threshold + order-preserving compaction   (cnn_detector_v3.cpp:1918-1947)
 //                  of candidate windows (levels in check order, raster order inside a level)
-//   k_check2       per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
-//                  resize to 49x51, u8, equalize, stage 2     image_proc.cpp:357-368,720-800)
-//   k_check3       stage 3 on the mirrored crop when needed
+//   k_check        per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
+//                  resize to 49x51, u8, equalize, stage 2,    image_proc.cpp:357-368,720-800)
+//                  stage 3 on the mirrored crop when needed
 //   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1
 
 #include <hip/hip_runtime.h>
@@ -696,17 +696,28 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
     s.hist[tid] = 0;
     __syncthreads();
-    for (int i = tid; i < CW * CH; i += CHECK_THREADS) {
+    // all of a thread's gathers are issued before any is used, so that their latencies overlap
+    constexpr int PER = (CW * CH + CHECK_THREADS - 1) / CHECK_THREADS;
+    float v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = min(tid + k * CHECK_THREADS, CW * CH - 1);
         const int iy = i / CW, ix = i - iy * CW;
         uint32_t px = ix, py = iy;
         if (!copy) {
             px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
             py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
         }
-        const float v = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
-        const uint32_t u = (uint32_t)(v + 0.5f) & 255u;
-        s.u8[i] = (uint8_t)u;
-        if (equalize) atomicAdd(&s.hist[u], 1);
+        v[k] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * CHECK_THREADS;
+        if (i < CW * CH) {
+            const uint32_t u = (uint32_t)(v[k] + 0.5f) & 255u;
+            s.u8[i] = (uint8_t)u;
+            if (equalize) atomicAdd(&s.hist[u], 1);
+        }
     }
     __syncthreads();
     if (equalize) {
@@ -726,24 +737,32 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     }
 }
 
-// Stage 2 of one (candidate, crop variant): crop preparation and the stage-2 network on
-// UCharToFloat(offset 2) (image_proc.cpp:416-427). grid = (workgroups, 2 crop variants).
-__global__ void __launch_bounds__(CHECK_THREADS) k_check2(const float* __restrict__ g, int gp, int gw, int gh,
-                                                          const LevelDesc* __restrict__ levels,
-                                                          const Candidate* __restrict__ cands,
-                                                          const int* __restrict__ meta, CheckParams P,
-                                                          const float* __restrict__ W2, int2* __restrict__ s2res)
+// Stages 2 and 3 of one (candidate, crop variant), grid = (workgroups, 2 crop variants):
+// crop preparation, the stage-2 network on UCharToFloat(offset 2) (image_proc.cpp:416-427),
+// then, unless the stage-2 outcome ends the check early (cnn_detector_v3.cpp:1704-1706),
+// the stage-3 network on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0.
+// s2res / s3res[2*ci + mod] = {count above the threshold, max response bits};
+// s3res {-1, -10.f} when stage 3 is skipped.
+__global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict__ g, int gp, int gw, int gh,
+                                                         const LevelDesc* __restrict__ levels,
+                                                         const Candidate* __restrict__ cands,
+                                                         const int* __restrict__ meta, CheckParams P,
+                                                         const float* __restrict__ W2, const float* __restrict__ W3,
+                                                         int2* __restrict__ s2res, int2* __restrict__ s3res)
 {
     using namespace chk;
     __shared__ ChkSmem s;
     __shared__ int s_w[4];
+    __shared__ int s_k1;
     const int tid = threadIdx.x;
     const int total = meta[0];
     const int mod = blockIdx.y;
     if (mod == 1 && !P.double_check) return;
     for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+        const size_t slot = (size_t)ci * 2 + mod;
         const Candidate cd = cands[ci];
-        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, mod, gw, gh);
+        const float scale = levels[cd.level].scale;
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gw, gh);
         prepare_crop(g, gp, cr, P.equalize != 0, s, s_w);
         for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
             const int yy = i / IW, xx = i - yy * IW;
@@ -759,43 +778,19 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check2(const float* __restric
                 m1 = (m1 < d) ? d : m1;
                 if (d > P.t2) k1++;
             }
-            s2res[(size_t)ci * 2 + mod] = make_int2(k1, __float_as_int(m1));
+            s2res[slot] = make_int2(k1, __float_as_int(m1));
+            s_k1 = k1;
         }
         __syncthreads();
-    }
-}
-
-// Stage 3 of one (candidate, crop variant) on UCharToFloat_inv (mirror, image_proc.cpp:
-// 478-490) or offset 0, run unless the stage-2 outcome ends the check early
-// (cnn_detector_v3.cpp:1704-1706). grid = (workgroups, 2).
-__global__ void __launch_bounds__(CHECK_THREADS) k_check3(const float* __restrict__ g, int gp, int gw, int gh,
-                                                          const LevelDesc* __restrict__ levels,
-                                                          const Candidate* __restrict__ cands,
-                                                          const int* __restrict__ meta, CheckParams P,
-                                                          const float* __restrict__ W3,
-                                                          const int2* __restrict__ s2res, int2* __restrict__ s3res)
-{
-    using namespace chk;
-    __shared__ ChkSmem s;
-    __shared__ int s_w[4];
-    const int tid = threadIdx.x;
-    const int total = meta[0];
-    const int mod = blockIdx.y;
-    if (mod == 1 && !P.double_check) return;
-    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
-        const size_t slot = (size_t)ci * 2 + mod;
-        const int k1 = s2res[slot].x;
-        const Candidate cd = cands[ci];
-        const float scale = levels[cd.level].scale;
+        const int k1 = s_k1;
         const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
         const bool skip3 = (P.type_check == 0 && k1 >= mn) || (P.type_check > 0 && k1 <= 0) ||
                            (P.type_check == 2 && k1 < mn);
         if (skip3) {
             if (tid == 0) s3res[slot] = make_int2(-1, __float_as_int(-10.f));
+            __syncthreads();
             continue;
         }
-        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gw, gh);
-        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w);
         for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
             const int yy = i / IW, xx = i - yy * IW;
             s.in[i] = (float)s.u8[yy * CW + (P.reflection ? 46 - xx : xx)];
@@ -1011,14 +1006,23 @@ __global__ void __launch_bounds__(FA_THREADS) k_facial(const float* __restrict__
         const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
         const uint32_t sclx = ((uint32_t)cr.rcols << 12) / fa::W + 1;
         const uint32_t scly = ((uint32_t)cr.rrows << 12) / fa::H + 1;
-        for (int i = tid; i < fa::W * fa::H; i += FA_THREADS) {
+        constexpr int PER = (fa::W * fa::H + FA_THREADS - 1) / FA_THREADS;
+        float v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = min(tid + k * FA_THREADS, fa::W * fa::H - 1);
             const int iy = i / fa::W, ix = i - iy * fa::W;
             uint32_t px = ix, py = iy;
             if (!copy) {
                 px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
                 py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
             }
-            s.in[i] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+            v[k] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = tid + k * FA_THREADS;
+            if (i < fa::W * fa::H) s.in[i] = v[k];
         }
         __syncthreads();
         landmarks_forward(W, s);
@@ -1128,8 +1132,7 @@ hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDe
                         int2* s3res, CheckResult* results, int nblocks, hipStream_t st)
 {
     dim3 grid(nblocks, 2);
-    k_check2<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, s2res);
-    k_check3<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W3, s2res, s3res);
+    k_check<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
     k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results);
     return hipGetLastError();
 }

@@ -114,6 +114,13 @@ WORKLOADS = {
                              max_face=240),
     "deep_4800x2400_sf105_min15": dict(w=4800, h=2400, lam=40.0, min_size=15, max_size=400, scale_factor=1.05,
                                        min_face=15, max_face=0),
+    # the mixed-size batch: 8 frames each of 640x480, 1280x720, 1920x1080 and 3840x2160
+    "mixed_vga": dict(w=640, h=480, lam=10.0, min_size=20, max_size=300, scale_factor=1.15, min_face=20, max_face=0),
+    "mixed_hd": dict(w=1280, h=720, lam=10.0, min_size=20, max_size=300, scale_factor=1.15, min_face=20, max_face=0),
+    "mixed_fhd": dict(w=1920, h=1080, lam=10.0, min_size=20, max_size=300, scale_factor=1.15, min_face=20,
+                      max_face=0),
+    "mixed_uhd": dict(w=3840, h=2160, lam=10.0, min_size=20, max_size=300, scale_factor=1.15, min_face=20,
+                      max_face=0),
 }
 
 

@@ -68,6 +68,12 @@ CASES = {
     "photo3_min20_facial": (photo3, dict(min_face=20, facial_analysis=1)),
     "photo3_min40_max200_facial_high": (photo3, dict(min_face=40, max_face=200, precision=3, facial_analysis=1)),
     "vga_s11_gray_min24_facial": (lambda: _gray(vga(11, smin=20)), dict(min_face=24, facial_analysis=1)),
+    # full-size frames of the mixed-size batch and the deep pyramid (goldens from the
+    # compiled reference; too slow for the oracle in the CPU suite)
+    "big_hd_mixed0_min20": (lambda: synth.workload_frame("mixed_hd", 0)[0], dict(min_face=20)),
+    "big_uhd_mixed0_min20": (lambda: synth.workload_frame("mixed_uhd", 0)[0], dict(min_face=20)),
+    "big_deep_4800x2400_sf105_min15": (lambda: synth.workload_frame("deep_4800x2400_sf105_min15", 0)[0],
+                                       dict(min_face=15, scale_factor=1.05)),
 }
 
 # 1-channel frames with min face >= 34 take the gray_image_only pre-resize path

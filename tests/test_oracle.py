@@ -42,8 +42,8 @@ def vectors():
 @pytest.mark.parametrize("name", list(CASES) + list(ORACLE_ONLY_CASES))
 def test_oracle_detections_match_golden(oracle, golden, name):
     build, over = {**CASES, **ORACLE_ONLY_CASES}[name]
-    if name.startswith("fhd"):
-        pytest.skip("1080p frames: covered by the GPU parity tests and the compiled reference")
+    if name.startswith("fhd") or name.startswith("big_"):
+        pytest.skip("large frames: covered by the GPU parity tests against the compiled reference's goldens")
     f = build()
     g = golden[name]
     assert frame_digest(f) == g["digest"]
