@@ -215,9 +215,11 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
                                                        const float* __restrict__ W, float* __restrict__ resp)
 {
     using namespace s1;
-    __shared__ float s_in[IH * IW];
-    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later reused for the L2 sums
-    __shared__ float s_p2[N::M2 * P2N];
+    static_assert(N::M2 * P2N <= IH * IW, "pooled L2 maps reuse the input window");
+    static_assert(TX * TY <= S1_THREADS, "one response value per thread");
+    __shared__ float s_in[IH * IW];         // input window; later the pooled L2 maps
+    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
+    float* s_p2 = s_in;
 
     const int tid = threadIdx.x;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -328,6 +330,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread.
+    if (tid >= TX * TY) return;             // no barrier follows
     const int ox = tid % TX, oy = tid / TX;
     const float* k3 = W + N::oK3;
     const float* b3 = W + N::oB3;
