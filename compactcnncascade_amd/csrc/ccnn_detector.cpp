@@ -156,6 +156,20 @@ int load_model(const unsigned char* bytes, size_t n, int stage, int index_output
     return fill(Landmarks{});
 }
 
+size_t level_l1_layout(LevelDesc& L)
+{
+    L.s1_w = L.s1_h = 0;
+    L.l1_tiles_x = L.l1_tiles_y = 0;
+    if (L.tiles_x <= 0 || L.tiles_y <= 0) return 0;
+    // the response tiles read pooled L1 rows 2*oy0 .. 2*oy0 + 2*TY + 9 and columns
+    // 2*ox0 .. 2*ox0 + 2*TX + 7
+    L.s1_w = 2 * S1_TX * L.tiles_x + 8;
+    L.s1_h = 2 * S1_TY * L.tiles_y + 10;
+    L.l1_tiles_x = (L.s1_w + S1A_TX - 1) / S1A_TX;
+    L.l1_tiles_y = (L.s1_h + S1A_TY - 1) / S1A_TY;
+    return (size_t)Stage1::M1 * L.s1_w * L.s1_h;
+}
+
 // ---------------------------------------------------------------------------------
 // Settings, scales
 // ---------------------------------------------------------------------------------
@@ -381,7 +395,8 @@ Detector::~Detector()
 void Detector::release_slot(Slot& s)
 {
     if (s.st) (void)hipStreamSynchronize(s.st);
-    dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_levels); dfree(s.d_counts);
+    dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_s1); dfree(s.d_levels);
+    dfree(s.d_counts);
     dfree(s.d_offsets); dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
     hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
     for (auto& e : s.ev)
@@ -490,8 +505,8 @@ const Plan* Detector::plan_for(int gw, int gh)
     Plan* P = new Plan();
     P->gw = gw;
     P->gh = gh;
-    int tiles = 0, chunks = 0;
-    size_t roff = 0;
+    int tiles = 0, chunks = 0, l1tiles = 0;
+    size_t roff = 0, s1off = 0;
     for (int scl = (int)scales_.size() - 1; scl >= 0; --scl) {
         LevelDesc L{};
         L.scale = scales_[scl];
@@ -512,6 +527,10 @@ const Plan* Detector::plan_for(int gw, int gh)
         L.tiles_x = (L.rw + S1_TX - 1) / S1_TX;
         L.tiles_y = (L.rh + S1_TY - 1) / S1_TY;
         tiles += L.tiles_x * L.tiles_y;
+        L.s1_off = (int)s1off;
+        L.l1_tile_off = l1tiles;
+        s1off += level_l1_layout(L);
+        l1tiles += L.l1_tiles_x * L.l1_tiles_y;
         L.chunk_off = chunks;
         L.chunks = (L.rw * L.rh + CMP_CHUNK - 1) / CMP_CHUNK;
         chunks += L.chunks;
@@ -519,7 +538,9 @@ const Plan* Detector::plan_for(int gw, int gh)
     }
     P->ntiles = tiles;
     P->nchunks = chunks;
+    P->nl1tiles = l1tiles;
     P->resp_total = roff;
+    P->s1_total = s1off;
     if (plans_.size() >= 8) {          // a handful of frame sizes stay cached
         for (size_t k = 0; k < plans_.size(); ++k) {
             bool used = false;
@@ -563,6 +584,10 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
             dalloc(s.d_s2, 2 * n) < 0 || dalloc(s.d_s3, 2 * n) < 0)
             return -1;
         s.resp_cap = s.cand_cap = n;
+    }
+    if (plan.s1_total > s.s1_cap) {
+        if (dalloc(s.d_s1, plan.s1_total) < 0) return -1;
+        s.s1_cap = plan.s1_total;
     }
     if ((size_t)plan.nchunks > s.chunk_cap) {
         if (dalloc(s.d_counts, plan.nchunks) < 0 || dalloc(s.d_offsets, plan.nchunks) < 0) return -1;
@@ -653,7 +678,8 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
         s.levels_plan = plan;
     }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
-    HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->ntiles, d_w_[0], s.d_resp, st));
+    HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
+                         s.d_resp, set_.t1, s.d_counts, plan->nchunks, st));
     if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
     HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_offsets, s.d_meta,
                           s.d_cands, st));

@@ -19,7 +19,11 @@ hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8
                             hipStream_t st);
 hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
-                         int ntiles, const float* W1, float* resp, hipStream_t st);
+                         int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
+                         int* counts, int nchunks, hipStream_t st);
+// L1-sum plane geometry and L1 tiles of a level whose response tiles are set
+// (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
+size_t level_l1_layout(LevelDesc& L);
 hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
                           int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
@@ -82,8 +86,9 @@ std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s);
 struct Plan {
     int gw = -1, gh = -1;
     std::vector<LevelDesc> levels;
-    int ntiles = 0, nchunks = 0;
+    int ntiles = 0, nchunks = 0, nl1tiles = 0;
     size_t resp_total = 0;
+    size_t s1_total = 0;         // floats of the L1-sum planes of all levels
 };
 
 // Device and pinned-host state of one frame in flight. Two slots on two streams let the
@@ -95,6 +100,7 @@ struct Slot {
     uint8_t* d_pre = nullptr;    size_t pre_bytes = 0;
     float* d_gray = nullptr;     size_t gray_elems = 0;  int gpitch = 0;
     float* d_resp = nullptr;     size_t resp_cap = 0;
+    float* d_s1 = nullptr;       size_t s1_cap = 0;
     LevelDesc* d_levels = nullptr;
     int* d_counts = nullptr;     int* d_offsets = nullptr; size_t chunk_cap = 0;
     int* d_meta = nullptr;

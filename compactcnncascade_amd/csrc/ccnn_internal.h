@@ -59,10 +59,17 @@ struct LevelDesc {
     int chunk_off;        // first compaction chunk of the level
     int chunks;           // compaction chunks
     float scale;          // pyramid scale
-    int pad;
+    int s1_off;           // offset of the level's L1-sum planes in the frame's L1 buffer
+    int s1_w, s1_h;       // L1-sum plane size: every pooled L1 position the tiles read
+    int l1_tile_off;      // first L1 tile of the level
+    int l1_tiles_x, l1_tiles_y;
 };
 
-// Stage-1 output tile: TX x TY response values per workgroup.
+// Stage 1 runs in two launches. The first computes the four L1-sum maps (conv, act,
+// pool, neighbour sums) of every level once, S1A_TX x S1A_TY pooled positions per
+// workgroup; the second computes TX x TY response values per workgroup from them.
+constexpr int S1A_TX = 64, S1A_TY = 16;
+constexpr int S1A_THREADS = 256;
 constexpr int S1_TX = 16, S1_TY = 16;
 constexpr int S1_THREADS = 256;
 // Compaction chunk (raster elements of one level per workgroup).

@@ -8,11 +8,14 @@
 //
 //   k_gray         u8 BGR/BGRA/gray -> FP32 gray             (image_proc.cpp:99-102,195-201,286-295)
 //   k_level        one pyramid level, materialised (tests)   (image_resize.cpp:46-122,328-541)
-//   k_stage1       fused pyramid sample + L1..OL per 16x16   (cnn_simd_cntk.cpp:640-963,
-//                  response tile, all levels in one launch    cnnpp_cplusplus.cpp)
-//   k_count / k_scan / k_scatter
-//                  threshold + order-preserving compaction   (cnn_detector_v3.cpp:1918-1947)
-//                  of candidate windows (levels in check order, raster order inside a level)
+//   k_stage1_l1    pyramid sampling + L1 + neighbour sums   (cnn_simd_cntk.cpp:640-725,
+//                  of every level, once                       cnnpp_cplusplus.cpp)
+//   k_stage1       per 16x16 response tile: L2, L3, hidden   (cnn_simd_cntk.cpp:726-963)
+//                  and output layers; counts the candidate
+//                  windows (score > t1) per compaction chunk
+//   k_scan / k_scatter
+//                  order-preserving compaction of the        (cnn_detector_v3.cpp:1918-1947)
+//                  candidate windows (levels in check order, raster order inside a level)
 //   k_check        per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
 //                  resize to 49x51, u8, equalize, stage 2,    image_proc.cpp:357-368,720-800)
 //                  stage 3 on the mirrored crop when needed
@@ -175,17 +178,23 @@ __global__ void __launch_bounds__(256) k_level(const float* __restrict__ g, int 
 }
 
 // ---------------------------------------------------------------------------------
-// Stage 1: fused level sampling + L1 (conv 4x4, act, pool) + sums + L2 (conv 3x3,
-// act, pool) + sums + L3 (conv 4w x 5h, act) + hidden layer + output layer.
-// One workgroup computes a 16x16 tile of one level's response map.
+// Stage 1 (cnn_simd_cntk.cpp:640-963, cnnpp_cplusplus.cpp), two launches per frame:
+//   k_stage1_l1: level sampling + L1 (conv 4x4, act, 2x2 max-pool) + the neighbour sums
+//                feeding L2, for every pooled L1 position of every level, once;
+//   k_stage1:    per 16x16 response tile, L2 (conv 3x3, act, pool) + sums + L3 (conv
+//                4w x 5h, act) + hidden layer + output layer from the L1 sums.
+// Each pooled L1 position depends only on the level samples at (2X..2X+4, 2Y..2Y+4),
+// so the split computes the same values as one fused pass without recomputing the
+// L1 halo that neighbouring response tiles share.
 // ---------------------------------------------------------------------------------
 namespace s1 {
 using N = Stage1;
 constexpr int TX = S1_TX, TY = S1_TY;
-constexpr int IW = 4 * TX + 19, IH = 4 * TY + 23;     // level input window
-constexpr int P1W = 2 * TX + 8, P1H = 2 * TY + 10;    // pooled L1 / first sums
+constexpr int P1W = 2 * TX + 8, P1H = 2 * TY + 10;    // L1 sums read by one response tile
 constexpr int P2W = TX + 3, P2H = TY + 4;             // pooled L2 / second sums
 constexpr int P1N = P1W * P1H, P2N = P2W * P2H;
+constexpr int AX = S1A_TX, AY = S1A_TY;               // pooled L1 positions per L1 tile
+constexpr int AW = 2 * AX + 3, AH = 2 * AY + 3;       // level window of one L1 tile
 }  // namespace s1
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): workgroup b
@@ -199,55 +208,56 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-__device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n, int t, bool chunks)
+// Level of global work item t: the last level whose first item (tile_off, chunk_off or
+// l1_tile_off) is <= t.
+enum { BY_TILE = 0, BY_CHUNK = 1, BY_L1_TILE = 2 };
+template <int BY>
+__device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n, int t)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int off = chunks ? L[mid].chunk_off : L[mid].tile_off;
+        const int off = BY == BY_TILE ? L[mid].tile_off : BY == BY_CHUNK ? L[mid].chunk_off : L[mid].l1_tile_off;
         if (off <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
 
-__global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__ g, int gp, int gw, int gh,
-                                                       const LevelDesc* __restrict__ levels, int nlevels,
-                                                       const float* __restrict__ W, float* __restrict__ resp)
+__global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restrict__ g, int gp, int gw, int gh,
+                                                           const LevelDesc* __restrict__ levels, int nlevels,
+                                                           const float* __restrict__ W, float* __restrict__ s1buf,
+                                                           int* __restrict__ counts, int nchunks)
 {
     using namespace s1;
-    static_assert(N::M2 * P2N <= IH * IW, "pooled L2 maps reuse the input window");
-    static_assert(TX * TY <= S1_THREADS, "one response value per thread");
-    __shared__ float s_in[IH * IW];         // input window; later the pooled L2 maps
-    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
-    float* s_p2 = s_in;
-
+    __shared__ float s_in[AH * AW];
     const int tid = threadIdx.x;
+    // clear the per-chunk candidate counts that k_stage1 accumulates
+    for (int c = blockIdx.x * S1A_THREADS + tid; c < nchunks; c += gridDim.x * S1A_THREADS) counts[c] = 0;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int li = find_level(levels, nlevels, tile, false);
+    const int li = find_level<BY_L1_TILE>(levels, nlevels, tile);
     const LevelDesc L = levels[li];
-    const int t = tile - L.tile_off;
-    const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
-    const int x0 = 4 * ox0, y0 = 4 * oy0;
+    const int t = tile - L.l1_tile_off;
+    const int X0 = (t % L.l1_tiles_x) * AX, Y0 = (t / L.l1_tiles_x) * AY;
+    const int x0 = 2 * X0, y0 = 2 * Y0;
 
-    // input window (clamped at the level border; clamped samples only feed outputs
-    // outside the response map, which are not stored). Samples go in batches of 8 per
+    // level window (clamped at the level border; clamped samples only feed L1 positions
+    // whose responses lie outside the response map). Samples go in batches of 8 per
     // thread so that their gray-image loads are in flight together.
     {
-        constexpr int NS = IW * IH, PER = (NS + S1_THREADS - 1) / S1_THREADS, B = 8;
+        constexpr int NS = AW * AH, PER = (NS + S1A_THREADS - 1) / S1A_THREADS, B = 8;
 #pragma unroll 1
         for (int b0 = 0; b0 < PER; b0 += B) {
             float v[B];
 #pragma unroll
             for (int k = 0; k < B; ++k) {
-                const int i = tid + (b0 + k) * S1_THREADS;
-                const int ii = min(i, NS - 1);
-                const int iy = ii / IW, ix = ii - iy * IW;
+                const int i = min(tid + (b0 + k) * S1A_THREADS, NS - 1);
+                const int iy = i / AW, ix = i - iy * AW;
                 const int lx = min(x0 + ix, L.lw - 1), ly = min(y0 + iy, L.lh - 1);
                 v[k] = level_px(g, gp, gw, gh, L, lx, ly);
             }
 #pragma unroll
             for (int k = 0; k < B; ++k) {
-                const int i = tid + (b0 + k) * S1_THREADS;
+                const int i = tid + (b0 + k) * S1A_THREADS;
                 if (i < NS) s_in[i] = v[k];
             }
         }
@@ -258,13 +268,17 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
     // (add / add2, cnnpp_cplusplus.cpp:916-950; wiring cnn_simd_cntk.cpp:705-725)
     const float* k1 = W + N::oK1;
     const float* b1 = W + N::oB1;
-    for (int p = tid; p < P1N; p += S1_THREADS) {
-        const int py = p / P1W, px = p - py * P1W;
+    const size_t plane = (size_t)L.s1_w * L.s1_h;
+    float* out = s1buf + L.s1_off;
+    for (int p = tid; p < AX * AY; p += S1A_THREADS) {
+        const int py = p / AX, px = p - py * AX;
+        const int X = X0 + px, Y = Y0 + py;
+        if (X >= L.s1_w || Y >= L.s1_h) continue;
         float win[5][5];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
 #pragma unroll
-            for (int c = 0; c < 5; ++c) win[r][c] = s_in[(2 * py + r) * IW + 2 * px + c];
+            for (int c = 0; c < 5; ++c) win[r][c] = s_in[(2 * py + r) * AW + 2 * px + c];
         float pm[N::M1];
 #pragma unroll
         for (int m = 0; m < N::M1; ++m) {
@@ -280,10 +294,53 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
             }
             pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
-        s_s1[0 * P1N + p] = pm[0] + pm[1];
+        float* o = out + (size_t)Y * L.s1_w + X;
+        o[0] = pm[0] + pm[1];
 #pragma unroll
-        for (int i = 1; i < N::M1 - 1; ++i) s_s1[i * P1N + p] = pm[i - 1] + pm[i] + pm[i + 1];
-        s_s1[(N::M1 - 1) * P1N + p] = pm[N::M1 - 2] + pm[N::M1 - 1];
+        for (int i = 1; i < N::M1 - 1; ++i) o[i * plane] = pm[i - 1] + pm[i] + pm[i + 1];
+        o[(N::M1 - 1) * plane] = pm[N::M1 - 2] + pm[N::M1 - 1];
+    }
+}
+
+__global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
+                                                       const float* __restrict__ W,
+                                                       const float* __restrict__ s1buf, float* __restrict__ resp,
+                                                       float t1, int* __restrict__ counts)
+{
+    using namespace s1;
+    static_assert(TX * TY <= S1_THREADS, "one response value per thread");
+    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
+    __shared__ float s_p2[N::M2 * P2N];
+
+    const int tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int li = find_level<BY_TILE>(levels, nlevels, tile);
+    const LevelDesc L = levels[li];
+    const int t = tile - L.tile_off;
+    const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
+
+    // the tile's L1 sums: rows 2*oy0 .. +P1H, columns 2*ox0 .. +P1W of each plane
+    // (inside the planes: s1_w = 2*TX*tiles_x + 8, s1_h = 2*TY*tiles_y + 10)
+    {
+        const size_t plane = (size_t)L.s1_w * L.s1_h;
+        const float* src = s1buf + L.s1_off + (size_t)(2 * oy0) * L.s1_w + 2 * ox0;
+        constexpr int NS = N::M1 * P1N, PER = (NS + S1_THREADS - 1) / S1_THREADS, B = 8;
+#pragma unroll 1
+        for (int b0 = 0; b0 < PER; b0 += B) {
+            float v[B];
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int i = min(tid + (b0 + k) * S1_THREADS, NS - 1);
+                const int m = i / P1N, r = i - m * P1N;
+                const int py = r / P1W, px = r - py * P1W;
+                v[k] = src[m * plane + (size_t)py * L.s1_w + px];
+            }
+#pragma unroll
+            for (int k = 0; k < B; ++k) {
+                const int i = tid + (b0 + k) * S1_THREADS;
+                if (i < NS) s_s1[i] = v[k];
+            }
+        }
     }
     __syncthreads();
 
@@ -330,8 +387,10 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread.
-    if (tid >= TX * TY) return;             // no barrier follows
+    // threads past the tile or outside the response map stop here (no barrier follows)
+    if (tid >= TX * TY) return;
     const int ox = tid % TX, oy = tid / TX;
+    if (ox0 + ox >= L.rw || oy0 + oy >= L.rh) return;
     const float* k3 = W + N::oK3;
     const float* b3 = W + N::oB3;
     float a3[N::M3];
@@ -388,8 +447,11 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const float* __restrict__
     c = rtanh(c);
     const float af = W[N::oOB + 2];
     const float out = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
-    const int gx = ox0 + ox, gy = oy0 + oy;
-    if (gx < L.rw && gy < L.rh) resp[L.resp_off + gy * L.rw + gx] = out;
+    const int e = (oy0 + oy) * L.rw + ox0 + ox;
+    resp[L.resp_off + e] = out;
+    // candidate windows per compaction chunk (raster order within the level), for the
+    // order-preserving scan that follows
+    if (out > t1) atomicAdd(&counts[L.chunk_off + e / CMP_CHUNK], 1);
 }
 
 // ---------------------------------------------------------------------------------
@@ -412,25 +474,6 @@ __device__ __forceinline__ int block_excl_scan_256(int v, int* s_tmp, int* total
     *total = s_tmp[255];
     __syncthreads();
     return incl - v;
-}
-
-__global__ void __launch_bounds__(256) k_count(const float* __restrict__ resp, const LevelDesc* __restrict__ levels,
-                                               int nlevels, float t1, int* __restrict__ counts)
-{
-    __shared__ int s_tmp[256];
-    const int c = blockIdx.x;
-    const int li = find_level(levels, nlevels, c, true);
-    const LevelDesc L = levels[li];
-    const int n = L.rw * L.rh;
-    const int base = (c - L.chunk_off) * CMP_CHUNK;
-    int cnt = 0;
-    for (int k = 0; k < CMP_CHUNK / 256; ++k) {
-        const int e = base + threadIdx.x * (CMP_CHUNK / 256) + k;
-        if (e < n && resp[L.resp_off + e] > t1) cnt++;
-    }
-    int total;
-    block_excl_scan_256(cnt, s_tmp, &total);
-    if (threadIdx.x == 0) counts[c] = total;
 }
 
 // Single workgroup: exclusive scan over the chunk counts; total written to meta[0].
@@ -464,7 +507,7 @@ __global__ void __launch_bounds__(256) k_scatter(const float* __restrict__ resp,
 {
     __shared__ int s_tmp[256];
     const int c = blockIdx.x;
-    const int li = find_level(levels, nlevels, c, true);
+    const int li = find_level<BY_CHUNK>(levels, nlevels, c);
     const LevelDesc L = levels[li];
     const int n = L.rw * L.rh;
     const int base = (c - L.chunk_off) * CMP_CHUNK;
@@ -668,8 +711,9 @@ __device__ __forceinline__ Crop crop_geometry(int px, int py, float scale, int m
     return c;
 }
 
-// Inclusive scan of one int per thread over a 256-thread block (4 waves of 64).
-__device__ __forceinline__ int block_incl_scan_256(int v, int* s_w)
+// Inclusive scan of one int per thread over the workgroup (waves of 64; s_w holds one
+// int per wave).
+__device__ __forceinline__ int block_incl_scan(int v, int* s_w)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -697,7 +741,8 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     const bool copy = (cr.rcols == CW && cr.rrows == CH);
     const uint32_t sclx = ((uint32_t)cr.rcols << 12) / CW + 1;
     const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
-    s.hist[tid] = 0;
+    static_assert(CHECK_THREADS >= 256 && CHECK_THREADS % 64 == 0, "one histogram bin per thread");
+    if (tid < 256) s.hist[tid] = 0;
     __syncthreads();
     // all of a thread's gathers are issued before any is used, so that their latencies overlap
     constexpr int PER = (CW * CH + CHECK_THREADS - 1) / CHECK_THREADS;
@@ -724,15 +769,15 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     }
     __syncthreads();
     if (equalize) {
-        const int cf = block_incl_scan_256(s.hist[tid], s_w);
-        s.lut[tid] = cf;
+        const int cf = block_incl_scan(tid < 256 ? s.hist[tid] : 0, s_w);
+        if (tid < 256) s.lut[tid] = cf;
         __syncthreads();
         const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
         if (bmin != bmax) {
             const float delt = 255.f / (bmax - bmin);
             const int v = (int)(((float)cf - bmin) * delt + 0.5f);
             __syncthreads();
-            s.lut[tid] = v;
+            if (tid < 256) s.lut[tid] = v;
             __syncthreads();
             for (int i = tid; i < CW * CH; i += CHECK_THREADS) s.u8[i] = (uint8_t)s.lut[s.u8[i]];
         }
@@ -755,7 +800,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
 {
     using namespace chk;
     __shared__ ChkSmem s;
-    __shared__ int s_w[4];
+    __shared__ int s_w[CHECK_THREADS / 64];
     __shared__ int s_k1;
     const int tid = threadIdx.x;
     const int total = meta[0];
@@ -1100,18 +1145,20 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 }
 
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
-                         int ntiles, const float* W1, float* resp, hipStream_t st)
+                         int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
+                         int* counts, int nchunks, hipStream_t st)
 {
     if (ntiles <= 0) return hipSuccess;
-    k_stage1<<<ntiles, S1_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, resp);
+    k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks);
+    k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, counts);
     return hipGetLastError();
 }
 
 hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
                           int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st)
 {
+    // counts: per-chunk candidate counts accumulated by k_stage1
     if (nchunks <= 0) return hipMemsetAsync(meta, 0, sizeof(int), st);
-    k_count<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, counts);
     k_scan<<<1, 1024, 0, st>>>(counts, nchunks, offsets, meta);
     k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, offsets, cands);
     return hipGetLastError();

@@ -1,5 +1,6 @@
 // Stage-level entry points of the C ABI (include/ccnn.h): each runs one kernel of the
 // detection path on host inputs and copies the result back.
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -114,16 +115,18 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out)
     L.lw = lw; L.lh = lh; L.rw = rw; L.rh = rh; L.mode = 2; L.scale = 1.f;
     L.tiles_x = (rw + ccnn::S1_TX - 1) / ccnn::S1_TX;
     L.tiles_y = (rh + ccnn::S1_TY - 1) / ccnn::S1_TY;
-    DevBuf g, w, r, l;
+    const size_t s1n = ccnn::level_l1_layout(L);
+    DevBuf g, w, r, l, s1;
     if (!g.alloc(sizeof(float) * (size_t)lw * (lh + 1)) || !w.alloc(sizeof(float) * net.w.size()) ||
-        !r.alloc(sizeof(float) * (size_t)rw * rh) || !l.alloc(sizeof(L)))
+        !r.alloc(sizeof(float) * (size_t)rw * rh) || !l.alloc(sizeof(L)) || !s1.alloc(sizeof(float) * s1n))
         return -1;
     if (hipMemcpy(g.p, level, sizeof(float) * (size_t)lw * lh, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(w.p, net.w.data(), sizeof(float) * net.w.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(l.p, &L, sizeof(L), hipMemcpyHostToDevice) != hipSuccess)
         return -1;
-    if (ccnn::launch_stage1(g.as<float>(), lw, lw, lh, l.as<ccnn::LevelDesc>(), 1, L.tiles_x * L.tiles_y,
-                            w.as<float>(), r.as<float>(), nullptr) != hipSuccess)
+    if (ccnn::launch_stage1(g.as<float>(), lw, lw, lh, l.as<ccnn::LevelDesc>(), 1, L.l1_tiles_x * L.l1_tiles_y,
+                            L.tiles_x * L.tiles_y, w.as<float>(), s1.as<float>(), r.as<float>(), INFINITY, nullptr,
+                            0, nullptr) != hipSuccess)
         return -1;
     if (hipMemcpy(out, r.p, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return 0;
