@@ -188,7 +188,7 @@ private:
     float last_ms_[5] = {0, 0, 0, 0, 0};
     double acc_ms_[5] = {0, 0, 0, 0, 0};
     int acc_frames_ = 0;
-    int check_blocks_ = 2048;
+    int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
     int fa_blocks_ = 256;          // one 118 KB-LDS workgroup per CU
 };
 

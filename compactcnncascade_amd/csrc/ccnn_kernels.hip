@@ -551,25 +551,28 @@ constexpr int P2N = P2W * P2H;
 constexpr int RW = P2W - 7 + 1, RH = P2H - 8 + 1;  // 4 x 4
 constexpr int RN = RW * RH;
 constexpr int HLMAX = 84;
+constexpr int WL = Stage2::SIZE > Stage3::SIZE ? Stage2::SIZE : Stage3::SIZE;
 }  // namespace chk
 
 struct ChkSmem {
-    float in[chk::IH * chk::IW];
+    float in[chk::IH * chk::IW];         // stage-2 input, later the stage-3 input
     float s1[6 * chk::P1N];
     float p2[12 * chk::P2N];
     float a3[24 * chk::RN];
     float h[chk::HLMAX * chk::RN];
-    float out[chk::RN];
+    alignas(16) float out[chk::RN];
+    float w[chk::WL];                    // weights of the running network (stage 2 or 3)
     int hist[256];
     int lut[256];
-    uint8_t u8[chk::CH * chk::CW + 3];
 };
 
 template <class N>
-__device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
+__device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid)
 {
     using namespace chk;
-    const int tid = threadIdx.x;
+    // the network's weights, staged in LDS (s.w); read the same for every thread in L1,
+    // per thread from L2 on
+    const float* Wl = s.w;
     // L1 + sums
     for (int p = tid; p < P1N; p += CHECK_THREADS) {
         const int py = p / P1W, px = p - py * P1W;
@@ -577,11 +580,11 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
 #pragma unroll
         for (int r = 0; r < 5; ++r)
 #pragma unroll
-            for (int c = 0; c < 5; ++c) win[r][c] = s.in[(2 * py + r) * IW + 2 * px + c];
+            for (int c = 0; c < 5; ++c) win[r][c] = in[(2 * py + r) * IW + 2 * px + c];
         float pm[N::M1];
 #pragma unroll
         for (int m = 0; m < N::M1; ++m) {
-            const float* k = W + N::oK1 + m * 16;
+            const float* k = Wl + N::oK1 + m * 16;
             float a[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -589,7 +592,7 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
                 float acc = win[dy][dx] * k[0];
 #pragma unroll
                 for (int j = 1; j < 16; ++j) acc = acc + win[dy + (j >> 2)][dx + (j & 3)] * k[j];
-                a[q] = lrelu_bn(acc, W + N::oB1, m, N::M1);
+                a[q] = lrelu_bn(acc, Wl + N::oB1, m, N::M1);
             }
             pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
@@ -612,7 +615,7 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int m = 2 * i + half;
-            const float* k = W + N::oK2 + m * 9;
+            const float* k = Wl + N::oK2 + m * 9;
             float a[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -620,7 +623,7 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
                 float acc = win[dy][dx] * k[0];
 #pragma unroll
                 for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
-                a[q] = lrelu_bn(acc, W + N::oB2, m, N::M2);
+                a[q] = lrelu_bn(acc, Wl + N::oB2, m, N::M2);
             }
             s.p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
@@ -645,12 +648,12 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int m = 2 * i + half;
-            const float* k = W + N::oK3 + m * N::K3W * N::K3H;
+            const float* k = Wl + N::oK3 + m * N::K3W * N::K3H;
             float acc = src[0] * k[0];
 #pragma unroll
             for (int j = 1; j < N::K3W * N::K3H; ++j)
                 acc = acc + src[(j / N::K3W) * P2W + (j % N::K3W)] * k[j];
-            s.a3[m * RN + p] = lrelu_bn(acc, W + N::oB3, m, N::M3);
+            s.a3[m * RN + p] = lrelu_bn(acc, Wl + N::oB3, m, N::M3);
         }
     }
     __syncthreads();
@@ -658,13 +661,13 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
     for (int it = tid; it < N::HL * RN; it += CHECK_THREADS) {
         const int u = it / RN, p = it - u * RN;
         const int gi = u / N::HLS, tt = u - gi * N::HLS;
-        float c = W[N::oHB + u];
+        float c = Wl[N::oHB + u];
 #pragma unroll
-        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[(gi + j) * RN + p] * W[N::oHW + u * N::HLN + j];
-        c = c * W[N::oHT + gi];
+        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[(gi + j) * RN + p] * Wl[N::oHW + u * N::HLN + j];
+        c = c * Wl[N::oHT + gi];
         c = rtanh(c);
         c = (c + 1.f) * 0.5f;
-        s.h[u * RN + p] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+        s.h[u * RN + p] = Wl[N::oHBW + tt] * c + Wl[N::oHBB + tt];
     }
     __syncthreads();
     // output layer
@@ -672,17 +675,30 @@ __device__ void check_forward(const float* __restrict__ W, ChkSmem& s)
         float o = 0.f;
         for (int gi = 0; gi < N::HLG; ++gi)
             for (int tt = 0; tt < N::HLS; ++tt) {
-                const float wv = W[N::oOW + tt * N::HLG + gi];
+                const float wv = Wl[N::oOW + tt * N::HLG + gi];
                 if (gi == 0 && tt == 0) o = s.h[tid] * wv;
                 else o = s.h[(N::HLS * gi + tt) * RN + tid] * wv + o;
             }
-        float c = o + W[N::oOB];
-        c = c * W[N::oOB + 1];
+        float c = o + Wl[N::oOB];
+        c = c * Wl[N::oOB + 1];
         c = rtanh(c);
-        const float af = W[N::oOB + 2];
+        const float af = Wl[N::oOB + 2];
         s.out[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     }
     __syncthreads();
+}
+
+// Copies n floats of weights into LDS with all of a thread's loads in flight together.
+template <int n>
+__device__ __forceinline__ void stage_weights(float* __restrict__ dst, const float* __restrict__ src, const int tid)
+{
+    constexpr int PER = (n + CHECK_THREADS - 1) / CHECK_THREADS;
+    float v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) v[k] = src[min(tid + k * CHECK_THREADS, n - 1)];
+#pragma unroll
+    for (int k = 0; k < PER; ++k)
+        if (tid + k * CHECK_THREADS < n) dst[tid + k * CHECK_THREADS] = v[k];
 }
 
 // Crop geometry of one candidate (cnn_detector_v3.cpp:1491-1556): the window around the
@@ -729,24 +745,29 @@ __device__ __forceinline__ int block_incl_scan(int v, int* s_w)
     return v + base;
 }
 
-// Crop of one (candidate, crop variant) into s.u8: NN resize to 49x51 (a copy at that
-// size), FloatToUChar (+0.5, truncate), then equalizeImage (image_proc.cpp:720-800):
-// cumulative histogram and LUT int((cf - cf[0]) * (255 / (cf[255] - cf[0])) + 0.5),
-// skipped if the histogram is flat.
+// Crop of one (candidate, crop variant): NN resize to 49x51 (a copy at that size,
+// image_resize.cpp:328-392, 564-584), FloatToUChar (+0.5, truncate, image_proc.cpp:
+// 357-368), then equalizeImage (image_proc.cpp:720-800: cumulative histogram and LUT
+// int((cf - cf[0]) * (255 / (cf[255] - cf[0])) + 0.5), skipped if the histogram is
+// flat). Each thread keeps its pixels (i = tid + k * CHECK_THREADS) in u[k]; when
+// remap is set, s.lut maps them to the equalized values.
+namespace chk {
+constexpr int PER = (CW * CH + CHECK_THREADS - 1) / CHECK_THREADS;
+}
+
 __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr, bool equalize, ChkSmem& s,
-                             int* s_w)
+                             int* s_w, uint32_t (&u)[chk::PER], bool& remap, const int tid)
 {
     using namespace chk;
-    const int tid = threadIdx.x;
     const bool copy = (cr.rcols == CW && cr.rrows == CH);
     const uint32_t sclx = ((uint32_t)cr.rcols << 12) / CW + 1;
     const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
     static_assert(CHECK_THREADS >= 256 && CHECK_THREADS % 64 == 0, "one histogram bin per thread");
     if (tid < 256) s.hist[tid] = 0;
     __syncthreads();
-    // all of a thread's gathers are issued before any is used, so that their latencies overlap
-    constexpr int PER = (CW * CH + CHECK_THREADS - 1) / CHECK_THREADS;
-    float v[PER];
+    // all of a thread's gathers are issued before any is used, so that their latencies
+    // overlap; 32-bit offsets from the crop's corner keep one register per address
+    const float* __restrict__ base = g + (size_t)cr.ry * gp + cr.rx;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = min(tid + k * CHECK_THREADS, CW * CH - 1);
@@ -756,36 +777,54 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
             px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
             py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
         }
-        v[k] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+        const float v = base[py * (uint32_t)gp + px];
+        u[k] = (uint32_t)(v + 0.5f) & 255u;
     }
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * CHECK_THREADS;
-        if (i < CW * CH) {
-            const uint32_t u = (uint32_t)(v[k] + 0.5f) & 255u;
-            s.u8[i] = (uint8_t)u;
-            if (equalize) atomicAdd(&s.hist[u], 1);
-        }
-    }
-    __syncthreads();
+    remap = false;
     if (equalize) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (tid + k * CHECK_THREADS < CW * CH) atomicAdd(&s.hist[u[k]], 1);
+        __syncthreads();
         const int cf = block_incl_scan(tid < 256 ? s.hist[tid] : 0, s_w);
         if (tid < 256) s.lut[tid] = cf;
         __syncthreads();
         const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
-        if (bmin != bmax) {
+        remap = bmin != bmax;
+        if (remap) {
             const float delt = 255.f / (bmax - bmin);
             const int v = (int)(((float)cf - bmin) * delt + 0.5f);
             __syncthreads();
             if (tid < 256) s.lut[tid] = v;
             __syncthreads();
-            for (int i = tid; i < CW * CH; i += CHECK_THREADS) s.u8[i] = (uint8_t)s.lut[s.u8[i]];
         }
-        __syncthreads();
     }
 }
 
-// Stages 2 and 3 of one (candidate, crop variant), grid = (workgroups, 2 crop variants):
+// A network input from the thread's crop pixels: stage 2 reads UCharToFloat(offset 2)
+// (image_proc.cpp:416-427, crop columns 2..48), stage 3 UCharToFloat_inv (mirror,
+// :478-490, columns 46..0) or UCharToFloat(offset 0) (columns 0..46).
+__device__ void write_input(ChkSmem& s, const uint32_t (&u)[chk::PER], bool remap, bool stage3, bool reflection,
+                            const int tid)
+{
+    using namespace chk;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * CHECK_THREADS;
+        if (i < CW * CH) {
+            const int iy = i / CW, ix = i - iy * CW;
+            const float f = (float)(remap ? (uint32_t)s.lut[u[k]] & 255u : u[k]);
+            if (!stage3) {
+                if (ix >= 2) s.in[iy * IW + ix - 2] = f;
+            } else if (ix <= 46) {
+                s.in[iy * IW + (reflection ? 46 - ix : ix)] = f;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Stages 2 and 3 of one (candidate, crop variant), one pair per workgroup at a time:
 // crop preparation, the stage-2 network on UCharToFloat(offset 2) (image_proc.cpp:416-427),
 // then, unless the stage-2 outcome ends the check early (cnn_detector_v3.cpp:1704-1706),
 // the stage-3 network on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0.
@@ -802,29 +841,42 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
     __shared__ ChkSmem s;
     __shared__ int s_w[CHECK_THREADS / 64];
     __shared__ int s_k1;
-    const int tid = threadIdx.x;
-    const int total = meta[0];
-    const int mod = blockIdx.y;
-    if (mod == 1 && !P.double_check) return;
-    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
+    // work items: (candidate, crop variant) pairs, both variants of a candidate next to
+    // each other so that they are dispatched together
+    const int nv = P.double_check ? 2 : 1;
+    const int items = meta[0] * nv;
+    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+        const int ci = nv == 2 ? item >> 1 : item;
+        const int mod = nv == 2 ? item & 1 : 0;
+        // the thread index is made opaque per candidate: otherwise every per-thread index
+        // of every phase is hoisted out of this loop and held in registers throughout,
+        // which caps the occupancy
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
         const size_t slot = (size_t)ci * 2 + mod;
         const Candidate cd = cands[ci];
         const float scale = levels[cd.level].scale;
         const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gw, gh);
-        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w);
-        for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
-            const int yy = i / IW, xx = i - yy * IW;
-            s.in[i] = (float)s.u8[yy * CW + xx + 2];
-        }
-        __syncthreads();
-        check_forward<Stage2>(W2, s);
+        // stage-2 weights into LDS; the first barrier of prepare_crop orders them
+        stage_weights<Stage2::SIZE>(s.w, W2, tid);
+        uint32_t u[PER];
+        bool remap;
+        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w, u, remap, tid);
+        write_input(s, u, remap, false, P.reflection != 0, tid);
+        check_forward<Stage2>(s.in, s, tid);
         if (tid == 0) {
+            float d[RN];
+#pragma unroll
+            for (int q = 0; q < RN / 4; ++q) {
+                const float4 v = reinterpret_cast<const float4*>(s.out)[q];
+                d[4 * q] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
+            }
             float m1 = -10.f;
             int k1 = 0;
+#pragma unroll
             for (int i = 0; i < RN; ++i) {
-                const float d = s.out[i];
-                m1 = (m1 < d) ? d : m1;
-                if (d > P.t2) k1++;
+                m1 = (m1 < d[i]) ? d[i] : m1;
+                if (d[i] > P.t2) k1++;
             }
             s2res[slot] = make_int2(k1, __float_as_int(m1));
             s_k1 = k1;
@@ -839,19 +891,23 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
             __syncthreads();
             continue;
         }
-        for (int i = tid; i < IW * IH; i += CHECK_THREADS) {
-            const int yy = i / IW, xx = i - yy * IW;
-            s.in[i] = (float)s.u8[yy * CW + (P.reflection ? 46 - xx : xx)];
-        }
-        __syncthreads();
-        check_forward<Stage3>(W3, s);
+        // stage-3 weights; the barrier closing write_input orders them
+        stage_weights<Stage3::SIZE>(s.w, W3, tid);
+        write_input(s, u, remap, true, P.reflection != 0, tid);
+        check_forward<Stage3>(s.in, s, tid);
         if (tid == 0) {
+            float d[RN];
+#pragma unroll
+            for (int q = 0; q < RN / 4; ++q) {
+                const float4 v = reinterpret_cast<const float4*>(s.out)[q];
+                d[4 * q] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
+            }
             float m2 = -10.f;
             int k2 = 0;
+#pragma unroll
             for (int i = 0; i < RN; ++i) {
-                const float d = s.out[i];
-                m2 = (m2 < d) ? d : m2;
-                if (d > P.t3) k2++;
+                m2 = (m2 < d[i]) ? d[i] : m2;
+                if (d[i] > P.t3) k2++;
             }
             s3res[slot] = make_int2(k2, __float_as_int(m2));
         }
@@ -1181,8 +1237,7 @@ hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDe
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, hipStream_t st)
 {
-    dim3 grid(nblocks, 2);
-    k_check<<<grid, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
+    k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
     k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results);
     return hipGetLastError();
 }
