@@ -73,14 +73,17 @@ def secondary_fps(api, torch, names, per_name, steps):
 
 
 def stage1_macs(levels):
-    """Algorithmic stage-1 MACs of one frame: per level 64 (L1) per conv-1 output,
-    72 (L2) per conv-2 output, 320 (L3) + 260 (hidden + output layer) per response."""
-    total = 0
+    """Algorithmic stage-1 MACs of one frame, split as the two stage-1 kernels compute
+    them: (L1: 64 per conv-1 output position, i.e. 4 maps x 16 taps; rest: 72 per
+    conv-2 output position (8 maps x 9 taps), 320 (L3) + 260 (hidden + output layer)
+    per response)."""
+    l1 = rest = 0
     for (_, lw, lh, rw, rh, _) in levels:
         c1, r1 = lw - 3, lh - 3
         c2, r2 = (c1 >> 1) - 2, (r1 >> 1) - 2
-        total += 64 * c1 * r1 + 72 * c2 * r2 + (320 + 260) * rw * rh
-    return total
+        l1 += 64 * c1 * r1
+        rest += 72 * c2 * r2 + (320 + 260) * rw * rh
+    return l1, rest
 
 
 STAGE2_MAC = 278496
@@ -221,7 +224,8 @@ def main():
     fps = nframes / dt
 
     # algorithmic work of the recorded pass
-    s1_mac = np.mean([stage1_macs(tr["levels"]) for tr in records])
+    s1 = np.array([stage1_macs(tr["levels"]) for tr in records], np.float64).mean(0)
+    s1_mac = s1.sum()
     e2 = e3 = 0
     cands = 0
     for tr in records:
@@ -231,13 +235,20 @@ def main():
         cands += len(tr["cand"])
     e2 /= len(records)
     e3 /= len(records)
-    per_frame_ms = {k: tm["totals"][k] / max(1, tm["frames"]) for k in ("gray", "stage1", "compact", "check")}
-    dominant = max(("stage1", "check"), key=lambda k: per_frame_ms[k])
-    if dominant == "stage1":
-        flops = 2.0 * s1_mac
-    else:
-        flops = 2.0 * (e2 * STAGE2_MAC + e3 * STAGE3_MAC)
-    achieved = flops / (per_frame_ms[dominant] * 1e-3) / 1e12
+    per_frame_ms = {k: tm["totals"][k] / max(1, tm["frames"])
+                    for k in ("gray", "stage1_l1", "stage1", "compact", "check", "decide")}
+    # algorithmic FLOP (2 per MAC) per launch of the three VALU kernels, and what their
+    # event-timed launches achieve
+    kflops = {"k_stage1_l1": 2.0 * s1[0], "k_stage1": 2.0 * s1[1],
+              "k_check": 2.0 * (e2 * STAGE2_MAC + e3 * STAGE3_MAC)}
+    kms = {"k_stage1_l1": per_frame_ms["stage1_l1"], "k_stage1": per_frame_ms["stage1"],
+           "k_check": per_frame_ms["check"]}
+    kernels = {k: {"ms": round(kms[k], 4), "gflop": round(kflops[k] / 1e9, 4),
+                   "tflops": round(kflops[k] / (kms[k] * 1e-3) / 1e12, 3),
+                   "frac_of_exact_valu": round(kflops[k] / (kms[k] * 1e-3) / 1e12 / VALU_EXACT_PEAK_TOPS, 3)}
+               for k in kflops}
+    dominant = max(kms, key=lambda k: kms[k])
+    achieved = kflops[dominant] / (kms[dominant] * 1e-3) / 1e12
     in_bytes = W * H * 3
     line = {
         "metric": "1080p frames/sec (plain FP32 cascade, bit-exact with the reference CPU path)",
@@ -257,17 +268,18 @@ def main():
                    "launch": "per-frame launches, consecutive frames pipelined over two streams, "
                              "frames resident in HBM"},
         "roofline": {
-            "kernel": "k_stage1" if dominant == "stage1" else "k_check",
+            "kernel": dominant,
             "bound": "mfma",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic("k_stage1" if dominant == "stage1" else "k_check", args.workload),
+            "traffic": measured_traffic(dominant, args.workload),
             "note": "FP32 VALU kernel on the exact path (separate mul/add, no packed FP32); its own ceiling "
                     "is %.1f T ops/s, frac of that %.3f" % (VALU_EXACT_PEAK_TOPS, achieved / VALU_EXACT_PEAK_TOPS),
         },
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in per_frame_ms.items()},
+        "valu_kernels": kernels,
         "work_per_frame": {"stage1_gmac": round(s1_mac / 1e9, 4), "candidates": round(cands / len(records), 1),
                            "stage2_evals": round(e2, 1), "stage3_evals": round(e3, 1)},
         "faces_per_frame": round(faces / nframes, 2),

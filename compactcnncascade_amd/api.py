@@ -297,12 +297,12 @@ class FaceDetector:
 
     def last_timing(self):
         """Per-kernel milliseconds of the last frame, and totals since set_profiling."""
-        ms = np.zeros(11, np.float32)
+        ms = np.zeros(15, np.float32)
         self._L.ccnn_last_timing(self._h, _ptr(ms))
-        keys = ("gray", "stage1", "compact", "check", "total")
+        keys = ("gray", "stage1_l1", "stage1", "compact", "check", "decide", "total")
         last = {k: float(ms[i]) for i, k in enumerate(keys)}
-        totals = {k: float(ms[5 + i]) for i, k in enumerate(keys)}
-        return dict(last=last, totals=totals, frames=int(ms[10]))
+        totals = {k: float(ms[7 + i]) for i, k in enumerate(keys)}
+        return dict(last=last, totals=totals, frames=int(ms[14]))
 
     def last_facial(self):
         """Facial analysis of the last frame's candidates: int array [n, 4] of

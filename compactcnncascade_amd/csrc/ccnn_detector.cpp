@@ -366,16 +366,21 @@ int dalloc(T*& p, size_t n)
     dfree(p);
     return hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
 }
+// pinned host memory mapped into the device address space; dp receives the device view
 template <class T>
-int halloc(T*& p, size_t n)
+int halloc(T*& p, T*& dp, size_t n)
 {
     hfree(p);
-    return hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
+    dp = nullptr;
+    if (hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
+        return -1;
+    return hipHostGetDevicePointer((void**)&dp, p, 0) == hipSuccess ? 0 : -1;
 }
 
-// candidates read back with the first copy; more need a second round trip
+// the pinned read-back buffers hold at least this many candidates
 constexpr size_t PREFETCH = 4096;
-// pinned read-back buffers are capped; larger candidate lists are copied to pageable memory
+// and at most this many; larger candidate lists are copied to pageable memory
 constexpr size_t PINNED_CAP = 1u << 20;
 
 }  // namespace
@@ -396,6 +401,7 @@ void Detector::release_slot(Slot& s)
 {
     if (s.st) (void)hipStreamSynchronize(s.st);
     dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_s1); dfree(s.d_levels);
+    dfree(s.d_l1_order);
     dfree(s.d_counts);
     dfree(s.d_offsets); dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
     hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
@@ -541,6 +547,19 @@ const Plan* Detector::plan_for(int gw, int gh)
     P->nl1tiles = l1tiles;
     P->resp_total = roff;
     P->s1_total = s1off;
+    // L1 tiles band by band: band j holds tile rows [j, j+1) * tiles_y / 8 of every level,
+    // so that XCD j (which runs one contiguous eighth of the grid) reads one eighth of
+    // the gray image for all levels
+    constexpr int NB = 8;
+    P->l1_order.reserve(l1tiles);
+    for (int j = 0; j < NB; ++j)
+        for (int l = 0; l < (int)P->levels.size(); ++l) {
+            const LevelDesc& L = P->levels[l];
+            for (int ty = 0; ty < L.l1_tiles_y; ++ty) {
+                if (ty * NB / L.l1_tiles_y != j) continue;
+                for (int tx = 0; tx < L.l1_tiles_x; ++tx) P->l1_order.push_back(make_int2(l, ty * L.l1_tiles_x + tx));
+            }
+        }
     if (plans_.size() >= 8) {          // a handful of frame sizes stay cached
         for (size_t k = 0; k < plans_.size(); ++k) {
             bool used = false;
@@ -576,8 +595,13 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
     }
     s.gpitch = gp;
     if (!s.d_levels && dalloc(s.d_levels, MAX_LEVELS) < 0) return -1;
+    if (plan.l1_order.size() > s.l1_order_cap) {
+        if (dalloc(s.d_l1_order, plan.l1_order.size()) < 0) return -1;
+        s.l1_order_cap = plan.l1_order.size();
+        s.levels_plan = nullptr;                 // re-upload the tables
+    }
     if (!s.d_meta && dalloc(s.d_meta, 16) < 0) return -1;
-    if (!s.h_meta && halloc(s.h_meta, 16) < 0) return -1;
+    if (!s.h_meta && halloc(s.h_meta, s.hd_meta, 16) < 0) return -1;
     if (plan.resp_total > s.resp_cap) {
         const size_t n = plan.resp_total;
         if (dalloc(s.d_resp, n) < 0 || dalloc(s.d_cands, n) < 0 || dalloc(s.d_results, n) < 0 ||
@@ -595,7 +619,7 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
     }
     const size_t hc = std::min(std::max<size_t>(plan.resp_total, PREFETCH), PINNED_CAP);
     if (hc > s.h_cap) {
-        if (halloc(s.h_cands, hc) < 0 || halloc(s.h_results, hc) < 0) return -1;
+        if (halloc(s.h_cands, s.hd_cands, hc) < 0 || halloc(s.h_results, s.hd_results, hc) < 0) return -1;
         s.h_cap = hc;
     }
     return 0;
@@ -675,30 +699,36 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
     const int nl = (int)plan->levels.size();
     if (s.levels_plan != plan && nl > 0) {
         HIP_OK(hipMemcpyAsync(s.d_levels, plan->levels.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, st));
+        if (!plan->l1_order.empty())
+            HIP_OK(hipMemcpyAsync(s.d_l1_order, plan->l1_order.data(), sizeof(int2) * plan->l1_order.size(),
+                                  hipMemcpyHostToDevice, st));
         s.levels_plan = plan;
     }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
-                         s.d_resp, set_.t1, s.d_counts, plan->nchunks, st));
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
+                         s.d_resp, set_.t1, s.d_counts, plan->nchunks, s.d_l1_order, st,
+                         profiling_ ? s.ev[2] : nullptr));
+    if (profiling_ && plan->ntiles <= 0) HIP_OK(hipEventRecord(s.ev[2], st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
     HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_offsets, s.d_meta,
                           s.d_cands, st));
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
     if (s.checks) {
         HIP_OK(launch_check(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), d_w_[1],
-                            d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_, st));
+                            d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_, st, profiling_ ? s.ev[5] : nullptr));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
                                  fa_blocks_, st));
+    } else if (profiling_) {
+        HIP_OK(hipEventRecord(s.ev[5], st));
     }
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
-    s.pre = std::min(std::min(s.h_cap, PREFETCH), s.cand_cap);
-    HIP_OK(hipMemcpyAsync(s.h_meta, s.d_meta, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[6], st));
+    // candidates and check results go straight to the mapped host buffers (up to s.pre)
+    s.pre = std::min(s.h_cap, s.cand_cap);
     if (nl > 0) {
-        HIP_OK(hipMemcpyAsync(s.h_cands, s.d_cands, sizeof(Candidate) * s.pre, hipMemcpyDeviceToHost, st));
-        if (s.checks)
-            HIP_OK(hipMemcpyAsync(s.h_results, s.d_results, sizeof(CheckResult) * s.pre, hipMemcpyDeviceToHost, st));
+        HIP_OK(launch_export(s.d_meta, s.d_cands, s.checks ? s.d_results : nullptr, s.hd_meta, s.hd_cands,
+                             s.hd_results, (int)s.pre, st));
     }
     s.plan = plan;
     s.gw = gw;
@@ -724,13 +754,8 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
     const Candidate* cands = s.h_cands;
     const CheckResult* res = s.h_results;
     if ((size_t)count > s.pre) {
-        if ((size_t)count <= s.h_cap) {
-            HIP_OK(hipMemcpy(s.h_cands + s.pre, s.d_cands + s.pre, sizeof(Candidate) * (count - s.pre),
-                             hipMemcpyDeviceToHost));
-            if (checks)
-                HIP_OK(hipMemcpy(s.h_results + s.pre, s.d_results + s.pre, sizeof(CheckResult) * (count - s.pre),
-                                 hipMemcpyDeviceToHost));
-        } else {
+        // more candidates than the mapped buffers hold: copy the whole list
+        {
             big_c.resize(count);
             HIP_OK(hipMemcpy(big_c.data(), s.d_cands, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
             cands = big_c.data();
@@ -742,9 +767,9 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         }
     }
     if (profiling_) {
-        for (int i = 0; i < 4; ++i) (void)hipEventElapsedTime(&last_ms_[i], s.ev[i], s.ev[i + 1]);
-        (void)hipEventElapsedTime(&last_ms_[4], s.ev[0], s.ev[4]);
-        for (int i = 0; i < 5; ++i) acc_ms_[i] += last_ms_[i];
+        for (int i = 0; i + 1 < Slot::NEV; ++i) (void)hipEventElapsedTime(&last_ms_[i], s.ev[i], s.ev[i + 1]);
+        (void)hipEventElapsedTime(&last_ms_[NT - 1], s.ev[0], s.ev[Slot::NEV - 1]);
+        for (int i = 0; i < NT; ++i) acc_ms_[i] += last_ms_[i];
         acc_frames_++;
     }
     last_count_ = count;
@@ -922,9 +947,9 @@ int Detector::gray_image(ccnn_image* img, int pipeline)
 
 int Detector::last_timing(float* ms) const
 {
-    for (int i = 0; i < 5; ++i) ms[i] = last_ms_[i];
-    for (int i = 0; i < 5; ++i) ms[5 + i] = (float)acc_ms_[i];
-    ms[10] = (float)acc_frames_;
+    for (int i = 0; i < NT; ++i) ms[i] = last_ms_[i];
+    for (int i = 0; i < NT; ++i) ms[NT + i] = (float)acc_ms_[i];
+    ms[2 * NT] = (float)acc_frames_;
     return 0;
 }
 

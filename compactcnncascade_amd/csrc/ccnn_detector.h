@@ -20,7 +20,7 @@ hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8
 hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, hipStream_t st);
+                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, hipEvent_t mid = nullptr);
 // L1-sum plane geometry and L1 tiles of a level whose response tiles are set
 // (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
 size_t level_l1_layout(LevelDesc& L);
@@ -28,10 +28,12 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
                           int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
-                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st);
+                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st, hipEvent_t mid = nullptr);
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
+hipError_t launch_export(const int* meta, const Candidate* cands, const CheckResult* results, int* hmeta,
+                         Candidate* hcands, CheckResult* hresults, int cap, hipStream_t st);
 
 // Last HIP error code seen by the library on this thread (0 if none); optionally reset.
 int last_error(bool reset);
@@ -89,28 +91,34 @@ struct Plan {
     int ntiles = 0, nchunks = 0, nl1tiles = 0;
     size_t resp_total = 0;
     size_t s1_total = 0;         // floats of the L1-sum planes of all levels
+    std::vector<int2> l1_order;  // L1 tiles {level, tile} in XCD-band order
 };
 
 // Device and pinned-host state of one frame in flight. Two slots on two streams let the
 // GPU work on frame i+1 while the host replays the decisions of frame i.
 struct Slot {
     hipStream_t st = nullptr;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // profiling events: start, after gray, after the L1 sums, after stage 1, after the
+    // compaction, after the stage-2/3 networks, after the check decisions
+    static constexpr int NEV = 7;
+    hipEvent_t ev[NEV] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint8_t* d_img = nullptr;    size_t img_bytes = 0;
     uint8_t* d_pre = nullptr;    size_t pre_bytes = 0;
     float* d_gray = nullptr;     size_t gray_elems = 0;  int gpitch = 0;
     float* d_resp = nullptr;     size_t resp_cap = 0;
     float* d_s1 = nullptr;       size_t s1_cap = 0;
     LevelDesc* d_levels = nullptr;
+    int2* d_l1_order = nullptr;  size_t l1_order_cap = 0;
     int* d_counts = nullptr;     int* d_offsets = nullptr; size_t chunk_cap = 0;
     int* d_meta = nullptr;
     Candidate* d_cands = nullptr;
     CheckResult* d_results = nullptr;
     int2* d_s2 = nullptr;        int2* d_s3 = nullptr;
     size_t cand_cap = 0;
-    int* h_meta = nullptr;
-    Candidate* h_cands = nullptr;
-    CheckResult* h_results = nullptr;
+    // mapped pinned host memory that k_export writes (h_*: host view, hd_*: device view)
+    int* h_meta = nullptr;            int* hd_meta = nullptr;
+    Candidate* h_cands = nullptr;     Candidate* hd_cands = nullptr;
+    CheckResult* h_results = nullptr; CheckResult* hd_results = nullptr;
     size_t h_cap = 0;
     const Plan* levels_plan = nullptr;   // plan whose level table is in d_levels
     // frame in flight
@@ -185,8 +193,9 @@ private:
     std::vector<LevelDesc> last_levels_;
 
     bool profiling_ = false;
-    float last_ms_[5] = {0, 0, 0, 0, 0};
-    double acc_ms_[5] = {0, 0, 0, 0, 0};
+    static constexpr int NT = Slot::NEV;         // 6 kernel spans + the whole frame
+    float last_ms_[NT] = {};
+    double acc_ms_[NT] = {};
     int acc_frames_ = 0;
     int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
     int fa_blocks_ = 256;          // one 118 KB-LDS workgroup per CU

@@ -226,17 +226,28 @@ __device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n
 __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restrict__ g, int gp, int gw, int gh,
                                                            const LevelDesc* __restrict__ levels, int nlevels,
                                                            const float* __restrict__ W, float* __restrict__ s1buf,
-                                                           int* __restrict__ counts, int nchunks)
+                                                           int* __restrict__ counts, int nchunks,
+                                                           const int2* __restrict__ order)
 {
     using namespace s1;
     __shared__ float s_in[AH * AW];
     const int tid = threadIdx.x;
     // clear the per-chunk candidate counts that k_stage1 accumulates
     for (int c = blockIdx.x * S1A_THREADS + tid; c < nchunks; c += gridDim.x * S1A_THREADS) counts[c] = 0;
+    // order (optional): the tiles as {level, tile within the level}, arranged so that the
+    // contiguous run each XCD takes covers one horizontal band of every level, whose
+    // gray-image rows then stay in that XCD's L2 across the levels
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int li = find_level<BY_L1_TILE>(levels, nlevels, tile);
+    int li, t;
+    if (order) {
+        const int2 o = order[tile];
+        li = o.x;
+        t = o.y;
+    } else {
+        li = find_level<BY_L1_TILE>(levels, nlevels, tile);
+        t = tile - levels[li].l1_tile_off;
+    }
     const LevelDesc L = levels[li];
-    const int t = tile - L.l1_tile_off;
     const int X0 = (t % L.l1_tiles_x) * AX, Y0 = (t / L.l1_tiles_x) * AY;
     const int x0 = 2 * X0, y0 = 2 * Y0;
 
@@ -1202,10 +1213,15 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, hipStream_t st)
+                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, hipEvent_t mid)
 {
     if (ntiles <= 0) return hipSuccess;
-    k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks);
+    k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks,
+                                                  l1_order);
+    if (mid) {
+        const hipError_t e = hipEventRecord(mid, st);
+        if (e != hipSuccess) return e;
+    }
     k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, counts);
     return hipGetLastError();
 }
@@ -1217,6 +1233,29 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
     if (nchunks <= 0) return hipMemsetAsync(meta, 0, sizeof(int), st);
     k_scan<<<1, 1024, 0, st>>>(counts, nchunks, offsets, meta);
     k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, offsets, cands);
+    return hipGetLastError();
+}
+
+// Read-back of a frame's candidate list and check results straight into mapped host
+// memory (no copy launches): the count to hmeta[0], the first min(count, cap) records.
+__global__ void __launch_bounds__(256) k_export(const int* __restrict__ meta, const Candidate* __restrict__ cands,
+                                                const CheckResult* __restrict__ results, int* __restrict__ hmeta,
+                                                Candidate* __restrict__ hcands, CheckResult* __restrict__ hresults,
+                                                int cap)
+{
+    const int total = meta[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = total;
+    const int n = min(total, cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        hcands[i] = cands[i];
+        if (results) hresults[i] = results[i];
+    }
+}
+
+hipError_t launch_export(const int* meta, const Candidate* cands, const CheckResult* results, int* hmeta,
+                         Candidate* hcands, CheckResult* hresults, int cap, hipStream_t st)
+{
+    k_export<<<32, 256, 0, st>>>(meta, cands, results, hmeta, hcands, hresults, cap);
     return hipGetLastError();
 }
 
@@ -1235,9 +1274,13 @@ hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStre
 
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
-                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st)
+                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st, hipEvent_t mid)
 {
     k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
+    if (mid) {
+        const hipError_t e = hipEventRecord(mid, st);
+        if (e != hipSuccess) return e;
+    }
     k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results);
     return hipGetLastError();
 }

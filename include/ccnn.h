@@ -163,10 +163,11 @@ int ccnn_last_facial(const ccnn_detector* d, int* out, int cap);
 int ccnn_last_error(int reset);
 
 /* Per-kernel timing from HIP events recorded on the stream each frame runs on, enabled
- * (and the totals reset) by ccnn_set_profiling(d, 1). ms[0..4] = gray (with upload and
- * pre-resize), stage 1, compaction, stage-2/3 checks, all, for the last frame; ms[5..9]
- * the same summed over the frames since profiling was enabled; ms[10] that frame count.
- * ms must hold 11 floats. Returns 0. */
+ * (and the totals reset) by ccnn_set_profiling(d, 1). ms[0..6] = gray (with upload and
+ * pre-resize), stage-1 L1 sums (k_stage1_l1), stage-1 tiles (k_stage1), compaction,
+ * stage-2/3 networks (k_check), check decisions and facial analysis, the whole frame,
+ * for the last frame; ms[7..13] the same summed over the frames since profiling was
+ * enabled; ms[14] that frame count. ms must hold 15 floats. Returns 0. */
 void ccnn_set_profiling(ccnn_detector* d, int on);
 int ccnn_last_timing(const ccnn_detector* d, float* ms);
 
