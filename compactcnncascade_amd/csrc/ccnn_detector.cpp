@@ -714,22 +714,26 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                           s.d_cands, st));
     if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
+    // candidates and check results go straight to the mapped host buffers (up to s.pre):
+    // written by the decision kernel, or by k_export after facial analysis or without checks
+    s.pre = std::min(s.h_cap, s.cand_cap);
+    const bool export_in_decide = s.checks && !set_.facial;
     if (s.checks) {
         HIP_OK(launch_check(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), d_w_[1],
-                            d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_, st, profiling_ ? s.ev[5] : nullptr));
+                            d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_,
+                            export_in_decide ? s.hd_meta : nullptr, export_in_decide ? s.hd_cands : nullptr,
+                            export_in_decide ? s.hd_results : nullptr, (int)s.pre, st,
+                            profiling_ ? s.ev[5] : nullptr));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
                                  fa_blocks_, st));
     } else if (profiling_) {
         HIP_OK(hipEventRecord(s.ev[5], st));
     }
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[6], st));
-    // candidates and check results go straight to the mapped host buffers (up to s.pre)
-    s.pre = std::min(s.h_cap, s.cand_cap);
-    if (nl > 0) {
+    if (nl > 0 && !export_in_decide)
         HIP_OK(launch_export(s.d_meta, s.d_cands, s.checks ? s.d_results : nullptr, s.hd_meta, s.hd_cands,
                              s.hd_results, (int)s.pre, st));
-    }
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[6], st));
     s.plan = plan;
     s.gw = gw;
     s.gh = gh;
@@ -1019,7 +1023,7 @@ int Detector::stage_check(const float* gray, int gw, int gh, int n, const float*
         hipMemcpy(d_c, cd.data(), sizeof(Candidate) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_m, &n, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
         launch_check(d_g, gp, gw, gh, d_l, d_c, d_m, check_params(), d_w_[1], d_w_[2], d_s2, d_s3, d_r,
-                     std::min(n, 2048), st) != hipSuccess ||
+                     std::min(2 * n, 2048), nullptr, nullptr, nullptr, 0, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(out.data(), d_r, sizeof(CheckResult) * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = -1;

@@ -28,7 +28,8 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
                           int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
-                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st, hipEvent_t mid = nullptr);
+                        int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
+                        CheckResult* hresults, int hcap, hipStream_t st, hipEvent_t mid = nullptr);
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);

@@ -197,6 +197,25 @@ constexpr int AX = S1A_TX, AY = S1A_TY;               // pooled L1 positions per
 constexpr int AW = 2 * AX + 3, AH = 2 * AY + 3;       // level window of one L1 tile
 }  // namespace s1
 
+__device__ __forceinline__ int block_excl_scan_256(int v, int* s_tmp, int* total)
+{
+    // exclusive scan of one int per thread over a 256-thread block
+    const int tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int a = tid >= off ? s_tmp[tid - off] : 0;
+        __syncthreads();
+        s_tmp[tid] += a;
+        __syncthreads();
+    }
+    const int incl = s_tmp[tid];
+    *total = s_tmp[255];
+    __syncthreads();
+    return incl - v;
+}
+
+
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): workgroup b
 // runs on XCD b % 8. Renumber them so that each XCD takes one contiguous run of tiles,
 // whose input windows overlap and share that XCD's L2. A bijection on [0, n).
@@ -399,9 +418,8 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread.
     // threads past the tile or outside the response map stop here (no barrier follows)
-    if (tid >= TX * TY) return;
     const int ox = tid % TX, oy = tid / TX;
-    if (ox0 + ox >= L.rw || oy0 + oy >= L.rh) return;
+    if (tid >= TX * TY || ox0 + ox >= L.rw || oy0 + oy >= L.rh) return;
     const float* k3 = W + N::oK3;
     const float* b3 = W + N::oB3;
     float a3[N::M3];
@@ -461,7 +479,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const int e = (oy0 + oy) * L.rw + ox0 + ox;
     resp[L.resp_off + e] = out;
     // candidate windows per compaction chunk (raster order within the level), for the
-    // order-preserving scan that follows
+    // order-preserving compaction
     if (out > t1) atomicAdd(&counts[L.chunk_off + e / CMP_CHUNK], 1);
 }
 
@@ -469,24 +487,6 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 // Candidate compaction: responses > t1, levels in check order (the level table is in
 // that order), raster order inside a level.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ int block_excl_scan_256(int v, int* s_tmp, int* total)
-{
-    // exclusive scan of one int per thread over a 256-thread block
-    const int tid = threadIdx.x;
-    s_tmp[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int a = tid >= off ? s_tmp[tid - off] : 0;
-        __syncthreads();
-        s_tmp[tid] += a;
-        __syncthreads();
-    }
-    const int incl = s_tmp[tid];
-    *total = s_tmp[255];
-    __syncthreads();
-    return incl - v;
-}
-
 // Single workgroup: exclusive scan over the chunk counts; total written to meta[0].
 __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int nchunks, int* __restrict__ offsets,
                                                int* __restrict__ meta)
@@ -932,9 +932,13 @@ __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restric
                                                       const Candidate* __restrict__ cands,
                                                       const int* __restrict__ meta, CheckParams P,
                                                       const int2* __restrict__ s2res, const int2* __restrict__ s3res,
-                                                      CheckResult* __restrict__ results)
+                                                      CheckResult* __restrict__ results, int* __restrict__ hmeta,
+                                                      Candidate* __restrict__ hcands,
+                                                      CheckResult* __restrict__ hresults, int hcap)
 {
+    // hmeta / hcands / hresults (optional): the read-back in mapped host memory, as k_export
     const int total = meta[0];
+    if (hmeta && blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = total;
     for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < total; ci += gridDim.x * blockDim.x) {
         const float scale = levels[cands[ci].level].scale;
         const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
@@ -958,6 +962,10 @@ __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restric
         res.fa.ok = 1;
         res.fa.gender = res.fa.smile = res.fa.glasses = -1;
         results[ci] = res;
+        if (hresults && ci < hcap) {
+            hresults[ci] = res;
+            hcands[ci] = cands[ci];
+        }
     }
 }
 
@@ -1274,14 +1282,16 @@ hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStre
 
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
-                        int2* s3res, CheckResult* results, int nblocks, hipStream_t st, hipEvent_t mid)
+                        int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
+                        CheckResult* hresults, int hcap, hipStream_t st, hipEvent_t mid)
 {
     k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
     if (mid) {
         const hipError_t e = hipEventRecord(mid, st);
         if (e != hipSuccess) return e;
     }
-    k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results);
+    k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
+                                                          hcands, hresults, hcap);
     return hipGetLastError();
 }
 
