@@ -407,6 +407,8 @@ void Detector::release_slot(Slot& s)
     hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
     for (auto& e : s.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    for (auto& e : s.kev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (s.st) { (void)hipStreamDestroy(s.st); s.st = nullptr; }
     s = Slot();
 }
@@ -423,6 +425,8 @@ int Detector::init_device()
     for (auto& s : slots_) {
         if (!s.st) HIP_OK(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
         for (auto& e : s.ev)
+            if (!e) HIP_OK(hipEventCreate(&e));
+        for (auto& e : s.kev)
             if (!e) HIP_OK(hipEventCreate(&e));
     }
     if (param.device_info) {
@@ -707,12 +711,12 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
     if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
                          s.d_resp, set_.t1, s.d_counts, plan->nchunks, s.d_l1_order, st,
-                         profiling_ ? s.ev[2] : nullptr));
-    if (profiling_ && plan->ntiles <= 0) HIP_OK(hipEventRecord(s.ev[2], st));
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
+                         profiling_ ? s.kev : nullptr));
+    if (profiling_ && plan->ntiles <= 0)
+        for (int k = 0; k < 4; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
     HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_offsets, s.d_meta,
                           s.d_cands, st));
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[4], st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
     // candidates and check results go straight to the mapped host buffers (up to s.pre):
     // written by the decision kernel, or by k_export after facial analysis or without checks
@@ -723,17 +727,17 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                             d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_,
                             export_in_decide ? s.hd_meta : nullptr, export_in_decide ? s.hd_cands : nullptr,
                             export_in_decide ? s.hd_results : nullptr, (int)s.pre, st,
-                            profiling_ ? s.ev[5] : nullptr));
+                            profiling_ ? s.kev + 4 : nullptr));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
                                  fa_blocks_, st));
     } else if (profiling_) {
-        HIP_OK(hipEventRecord(s.ev[5], st));
+        for (int k = 4; k < 6; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
     }
     if (nl > 0 && !export_in_decide)
         HIP_OK(launch_export(s.d_meta, s.d_cands, s.checks ? s.d_results : nullptr, s.hd_meta, s.hd_cands,
                              s.hd_results, (int)s.pre, st));
-    if (profiling_) HIP_OK(hipEventRecord(s.ev[6], st));
+    if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
     s.plan = plan;
     s.gw = gw;
     s.gh = gh;
@@ -771,8 +775,12 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         }
     }
     if (profiling_) {
-        for (int i = 0; i + 1 < Slot::NEV; ++i) (void)hipEventElapsedTime(&last_ms_[i], s.ev[i], s.ev[i + 1]);
-        (void)hipEventElapsedTime(&last_ms_[NT - 1], s.ev[0], s.ev[Slot::NEV - 1]);
+        // gray, k_stage1_l1, k_stage1, compaction, k_check, decisions (+ facial analysis,
+        // read-back), the whole frame
+        const hipEvent_t span[NT][2] = {{s.ev[0], s.ev[1]},  {s.kev[0], s.kev[1]}, {s.kev[2], s.kev[3]},
+                                        {s.kev[3], s.ev[2]}, {s.kev[4], s.kev[5]}, {s.kev[5], s.ev[3]},
+                                        {s.ev[0], s.ev[3]}};
+        for (int i = 0; i < NT; ++i) (void)hipEventElapsedTime(&last_ms_[i], span[i][0], span[i][1]);
         for (int i = 0; i < NT; ++i) acc_ms_[i] += last_ms_[i];
         acc_frames_++;
     }

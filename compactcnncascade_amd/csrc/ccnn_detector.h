@@ -20,7 +20,8 @@ hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8
 hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, hipEvent_t mid = nullptr);
+                         int* counts, int nchunks, const int2* l1_order, hipStream_t st,
+                         const hipEvent_t* kev = nullptr);
 // L1-sum plane geometry and L1 tiles of a level whose response tiles are set
 // (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
 size_t level_l1_layout(LevelDesc& L);
@@ -29,7 +30,7 @@ hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlev
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
-                        CheckResult* hresults, int hcap, hipStream_t st, hipEvent_t mid = nullptr);
+                        CheckResult* hresults, int hcap, hipStream_t st, const hipEvent_t* kev = nullptr);
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
@@ -99,10 +100,12 @@ struct Plan {
 // GPU work on frame i+1 while the host replays the decisions of frame i.
 struct Slot {
     hipStream_t st = nullptr;
-    // profiling events: start, after gray, after the L1 sums, after stage 1, after the
-    // compaction, after the stage-2/3 networks, after the check decisions
-    static constexpr int NEV = 7;
-    hipEvent_t ev[NEV] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // profiling events: on the stream at the frame's start, after gray, after the
+    // compaction and at the end (ev); start/stop of k_stage1_l1, k_stage1 and k_check,
+    // stamped by the launches themselves (kev)
+    static constexpr int NEV = 4, NKEV = 6;
+    hipEvent_t ev[NEV] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t kev[NKEV] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint8_t* d_img = nullptr;    size_t img_bytes = 0;
     uint8_t* d_pre = nullptr;    size_t pre_bytes = 0;
     float* d_gray = nullptr;     size_t gray_elems = 0;  int gpitch = 0;
@@ -194,7 +197,7 @@ private:
     std::vector<LevelDesc> last_levels_;
 
     bool profiling_ = false;
-    static constexpr int NT = Slot::NEV;         // 6 kernel spans + the whole frame
+    static constexpr int NT = 7;                 // 6 kernel spans + the whole frame
     float last_ms_[NT] = {};
     double acc_ms_[NT] = {};
     int acc_frames_ = 0;

@@ -22,6 +22,7 @@
 //   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include "ccnn_internal.h"
 
 namespace ccnn {
@@ -1221,16 +1222,22 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, hipEvent_t mid)
+                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, const hipEvent_t* kev)
 {
     if (ntiles <= 0) return hipSuccess;
-    k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks,
-                                                  l1_order);
-    if (mid) {
-        const hipError_t e = hipEventRecord(mid, st);
-        if (e != hipSuccess) return e;
-    }
-    k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, counts);
+    // kev (optional): start/stop events of the two launches, stamped when each kernel
+    // starts and ends executing
+    if (kev)
+        hipExtLaunchKernelGGL(k_stage1_l1, dim3(nl1tiles), dim3(S1A_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw,
+                              gh, d_levels, nlevels, W1, s1buf, counts, nchunks, l1_order);
+    else
+        k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks,
+                                                      l1_order);
+    if (kev)
+        hipExtLaunchKernelGGL(k_stage1, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels, nlevels, W1,
+                              s1buf, resp, t1, counts);
+    else
+        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, counts);
     return hipGetLastError();
 }
 
@@ -1283,13 +1290,14 @@ hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStre
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
-                        CheckResult* hresults, int hcap, hipStream_t st, hipEvent_t mid)
+                        CheckResult* hresults, int hcap, hipStream_t st, const hipEvent_t* kev)
 {
-    k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
-    if (mid) {
-        const hipError_t e = hipEventRecord(mid, st);
-        if (e != hipSuccess) return e;
-    }
+    // kev (optional): start/stop events of k_check, stamped when it starts and ends executing
+    if (kev)
+        hipExtLaunchKernelGGL(k_check, dim3(nblocks), dim3(CHECK_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw, gh,
+                              d_levels, cands, meta, P, W2, W3, s2res, s3res);
+    else
+        k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
     k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
                                                           hcands, hresults, hcap);
     return hipGetLastError();
