@@ -265,7 +265,7 @@ def main():
         "data": "synthetic (seeded generator, compactcnncascade_amd/synth.py)",
         "config": {"workload": args.workload, "frames_per_step": args.frames, "frame": "%dx%d BGR u8" % (W, H),
                    "scale_factor": cfg["scale_factor"], "min_face": cfg["min_face"], "max_face": cfg["max_face"],
-                   "launch": "per-frame launches, consecutive frames pipelined over two streams, "
+                   "launch": "per-frame launches, consecutive frames pipelined over three streams, "
                              "frames resident in HBM"},
         "roofline": {
             "kernel": dominant,

@@ -911,30 +911,41 @@ int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_fa
 int Detector::detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces)
 {
     faces.assign(std::max(n, 0), std::vector<ccnn_face>());
-    int pending = -1;            // frame index in flight on slots_[pending % NSLOTS]
+    // frames go to the slots round robin; when every slot holds a frame in flight, the
+    // oldest is finished (host replay) while the others run on the GPU
+    int owner[NSLOTS];
+    for (int& o : owner) o = -1;
+    int next = 0;
+    auto finish_slot = [&](int k) -> int {
+        const int f = owner[k];
+        owner[k] = -1;
+        return f >= 0 ? finish(slots_[k], faces[f]) : 0;
+    };
+    auto drain = [&]() -> int {
+        for (int j = 0; j < NSLOTS; ++j)
+            if (finish_slot((next + j) % NSLOTS) < 0) return -1;
+        return 0;
+    };
     for (int i = 0; i < n; ++i) {
-        // the guards may re-initialise the detector: drain the frame in flight first
+        // the guards may re-initialise the detector: finish the frames in flight first
         const bool guard = empty_ || !built_ || imgs[i].cols > param.max_image_width ||
                            imgs[i].rows > param.max_image_height ||
                            (gray_only_ && pre_scale_ > 0.f && imgs[i].channels != 1);
-        if (guard && pending >= 0) {
-            if (finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
-            pending = -1;
-        }
+        if (guard && drain() < 0) return -1;
         const int r = prepare(imgs[i]);
         if (r < 0) {
             faces[i].clear();
             continue;
         }
         if (r == 0) continue;
-        Slot& s = slots_[i % NSLOTS];
+        if (finish_slot(next) < 0) return -1;
+        Slot& s = slots_[next];
         if (s.busy) (void)hipStreamSynchronize(s.st);
         if (enqueue(s, imgs[i], device_ptr) < 0) return -1;
-        if (pending >= 0 && finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
-        pending = i;
+        owner[next] = i;
+        next = (next + 1) % NSLOTS;
     }
-    if (pending >= 0 && finish(slots_[pending % NSLOTS], faces[pending]) < 0) return -1;
-    return 0;
+    return drain();
 }
 
 int Detector::gray_image(ccnn_image* img, int pipeline)

@@ -96,7 +96,7 @@ struct Plan {
     std::vector<int2> l1_order;  // L1 tiles {level, tile} in XCD-band order
 };
 
-// Device and pinned-host state of one frame in flight. Two slots on two streams let the
+// Device and pinned-host state of one frame in flight. Three slots on three streams let the
 // GPU work on frame i+1 while the host replays the decisions of frame i.
 struct Slot {
     hipStream_t st = nullptr;
@@ -134,14 +134,14 @@ struct Slot {
 
 class Detector {
 public:
-    static constexpr int NSLOTS = 2;
+    static constexpr int NSLOTS = 3;
     Detector();
     ~Detector();
 
     int init(const ccnn_param& p);
     int detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces);
     // Detection of n frames with the per-frame launches of consecutive frames pipelined
-    // over two streams. faces[i] receives frame i's detections; returns 0, or -1 on a
+    // over the slots' streams. faces[i] receives frame i's detections; returns 0, or -1 on a
     // HIP error or an uninitialised detector.
     int detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
     int clear();

@@ -88,8 +88,8 @@ int ccnn_detect_device(ccnn_detector* d, ccnn_face* faces, int capacity, const c
 
 /* Detection of n frames (host images, or device pointers when device != 0), each with
  * the semantics of one ccnn_detect call, the launches of consecutive frames pipelined
- * over two GPU streams so that the host-side decisions of frame i overlap the GPU work
- * of frame i+1. Frame i's faces go to faces[i*capacity ...] (at most capacity) and its
+ * over three GPU streams so that the host-side decisions of frame i overlap the GPU work
+ * of frames i+1 and i+2. Frame i's faces go to faces[i*capacity ...] (at most capacity) and its
  * face count to counts[i]. Returns 0, or -1. No counterpart in the reference, whose
  * Detect takes one frame per call. */
 int ccnn_detect_batch(ccnn_detector* d, int n, const ccnn_image* imgs, int device, ccnn_face* faces,
