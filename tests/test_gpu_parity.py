@@ -254,3 +254,48 @@ def test_batch_matches_single_frame_results(api, golden):
         singles.append(d3.Detect(f))
     for (cb, fb), (cs, fs) in zip(batch, singles):
         assert cb == cs and np.array_equal(fb, fs)
+
+
+def test_gray_image_of_last_frame(api, oracle):
+    # getGrayImage32F (CompactCNNLibAPI_v2.cpp:462-507): the FP32 gray image the last
+    # Detect worked on, as a host copy (Pipeline::CPU) and as a device pointer
+    from cases import vga
+    f = vga(2)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, dict(min_face=40))) == 0
+    d.Detect(f)
+    g = d.getGrayImage32F(api.PIPELINE_CPU)
+    assert g is not None and g.shape == f.shape[:2]
+    assert np.array_equal(bits(g), bits(oracle.gray(f)))
+    dev = d.getGrayImage32F(api.PIPELINE_GPU)
+    assert dev is not None and dev.cols == f.shape[1] and dev.rows == f.shape[0] and dev.data
+    # a 1-channel frame with min face >= 34 is pre-resized: the gray image is the resized one
+    gray_frame = np.ascontiguousarray(f[:, :, 1])
+    d2 = api.FaceDetector()
+    assert d2.Init(api_param(gray_frame, dict(min_face=40))) == 0
+    d2.Detect(gray_frame)
+    g2 = d2.getGrayImage32F(api.PIPELINE_CPU)
+    sw, sh = int(640 * np.float32(27) / np.float32(40)), int(480 * np.float32(27) / np.float32(40))
+    assert g2.shape == (sh, sw)
+    exp = oracle.resize(gray_frame, sw, sh, 1).astype(np.float32)
+    assert np.array_equal(bits(g2), bits(exp))
+
+
+def test_parameter_setters_and_clear(api):
+    from cases import vga
+    f = vga(0)
+    d = api.FaceDetector()
+    assert d.isEmpty()
+    assert d.Init(api_param(f, dict(min_face=40))) == 0
+    assert not d.isEmpty()
+    n40, faces40 = d.Detect(f)
+    # setMinObjectHeight re-initialises the detector like FaceDetector::setMinObjectHeight
+    d.setMinObjectHeight(80)
+    assert d.getMinObjectHeight() == 80
+    n80, _ = d.Detect(f)
+    d.setMinObjectHeight(40)
+    again = d.Detect(f)
+    assert again[0] == n40 and np.array_equal(again[1], faces40)
+    assert d.Clear() == 0 and d.isEmpty()
+    assert d.Clear() == -1
+    assert d.Detect(f)[0] == -1
