@@ -461,11 +461,9 @@ int FaceDetector::getGrayImage32F(ImageData* img, Pipeline pipeline)
     return 0;
 }
 
-void FaceDetector::CNTKDump2Binary(const char*, const char*)
+void FaceDetector::CNTKDump2Binary(const char* binary_file, const char* cntk_model_dump)
 {
-    // The CNTK-dump converter (cnn_models_converter.cpp) is an offline training-side tool
-    // and not part of this detection library.
-    std::printf("[CompactCNNLibAPI] CNTKDump2Binary is not provided by this build\n");
+    ccnn_cntk_dump_to_binary(binary_file, cntk_model_dump);
 }
 
 }  // namespace CompactCNNLib

@@ -115,6 +115,13 @@ void ccnn_set_num_threads(ccnn_detector* d, int n);
  * pipeline CCNN_PIPELINE_CPU: host pointer (library-owned copy); otherwise a device
  * pointer. step is in elements, as the reference reports widthStep. 0 or -1. */
 int ccnn_get_gray_image(ccnn_detector* d, ccnn_image* img, int pipeline);
+/* FaceDetector::CNTKDump2Binary (:509-519; CNNModelsConverter::LoadCNTKModel and
+ * SaveToBinaryFile, cnn_models_converter.cpp:270-742): converts the text dump of a
+ * trained CNTK cascade network into the binary model format v1.1, byte-identical to the
+ * reference's output (".bin" is appended to binary_file when it has none). Host only;
+ * needs no detector and no GPU. 0, or -1 when the dump cannot be read or is not a
+ * three-layer cascade network (then no file is written). */
+int ccnn_cntk_dump_to_binary(const char* binary_file, const char* cntk_model_dump);
 
 /* ---------------------------------------------------------------------------------
  * Stage-level entry points. They run one kernel of the detection path on host

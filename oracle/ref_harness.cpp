@@ -13,6 +13,7 @@
 //   ref_check_prep  -> FloatToUChar + equalizeImage + UCharToFloat(_inv) (image_proc.cpp:328-491,720-800)
 //   ref_detect      -> CNNDetector::Detect with FaceDetector's parameter conversion
 //                      (CompactCNNLibAPI_v2.cpp:100-138, cnn_detector_v3.cpp:2483-2813)
+//   ref_cntk_dump2binary -> FaceDetector::CNTKDump2Binary       (CompactCNNLibAPI_v2.cpp:509-519)
 
 #include "CompactCNNLibAPI_v2.h"
 #include "cnn_detector_v3.h"
@@ -267,6 +268,11 @@ long ref_detect_loop(const int* iparams, const float* fparams,
         }
     delete d;
     return total;
+}
+
+void ref_cntk_dump2binary(const char* binary_file, const char* cntk_model_dump)
+{
+    CompactCNNLib::FaceDetector::CNTKDump2Binary(binary_file, cntk_model_dump);
 }
 
 }  // extern "C"
