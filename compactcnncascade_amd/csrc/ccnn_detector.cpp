@@ -165,8 +165,11 @@ size_t level_l1_layout(LevelDesc& L)
     // 2*ox0 .. 2*ox0 + 2*TX + 7
     L.s1_w = 2 * S1_TX * L.tiles_x + 8;
     L.s1_h = 2 * S1_TY * L.tiles_y + 10;
-    L.l1_tiles_x = (L.s1_w + S1A_TX - 1) / S1A_TX;
-    L.l1_tiles_y = (L.s1_h + S1A_TY - 1) / S1A_TY;
+    // the L1 tiles cover the positions that responses inside the map read: columns
+    // < 2*rw + 8, rows < 2*rh + 10. The rest of the planes (read only by the tiles'
+    // threads past the response map, whose results are dropped) is never written.
+    L.l1_tiles_x = (2 * L.rw + 8 + S1A_TX - 1) / S1A_TX;
+    L.l1_tiles_y = (2 * L.rh + 10 + S1A_TY - 1) / S1A_TY;
     return (size_t)Stage1::M1 * L.s1_w * L.s1_h;
 }
 
@@ -593,6 +596,8 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
     }
     const int gp = (w + 63) / 64 * 64;
     const size_t ge = (size_t)gp * (h + 1);
+    // the stage-1 sampler addresses the gray image with 32-bit element offsets
+    if (ge > 0xFFFFFFFFull) return -1;
     if (ge > s.gray_elems || gp > s.gpitch) {
         if (dalloc(s.d_gray, std::max(ge, s.gray_elems)) < 0) return -1;
         s.gray_elems = std::max(ge, s.gray_elems);
@@ -615,6 +620,9 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
     }
     if (plan.s1_total > s.s1_cap) {
         if (dalloc(s.d_s1, plan.s1_total) < 0) return -1;
+        // plane positions no L1 tile writes then hold zeros rather than whatever the
+        // allocation held
+        if (hipMemset(s.d_s1, 0, plan.s1_total * sizeof(float)) != hipSuccess) return -1;
         s.s1_cap = plan.s1_total;
     }
     if ((size_t)plan.nchunks > s.chunk_cap) {

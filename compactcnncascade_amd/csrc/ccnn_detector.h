@@ -34,6 +34,7 @@ hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDe
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
+hipError_t launch_selftest_tanh(uint32_t lo, uint32_t hi, unsigned long long* out, hipStream_t st);
 hipError_t launch_export(const int* meta, const Candidate* cands, const CheckResult* results, int* hmeta,
                          Candidate* hcands, CheckResult* hresults, int cap, hipStream_t st);
 

@@ -27,6 +27,22 @@
 
 namespace ccnn {
 
+// The reciprocal of the rational tanh's denominator d >= 1, as 1 - 1/d needs it: 1/d
+// rounded to nearest for 1 <= d < 2^25, from the hardware reciprocal estimate (within
+// 1 ulp) and one Newton-Raphson step r + r(1 - d r) evaluated with fused multiply-adds
+// (Markstein's correction); 0 for d >= 2^25 (and +inf), where 1/d <= 2^-25 and 1 - 1/d
+// rounds to 1 either way. A NaN stays a NaN. Five instructions instead of the general
+// IEEE division's scale/fixup sequence; ccnn_selftest_tanh checks the reciprocal against
+// IEEE division on every float in [1, 2^25) and the tanh on every float.
+__device__ __forceinline__ float recip_ge1(float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.f);
+    const float q = __builtin_fmaf(e, r, r);
+    return d >= 33554432.f ? 0.f : q;
+}
+
+template <bool IEEE_DIV = false>
 __device__ __forceinline__ float rtanh(float c)
 {
     // sgn * (1 - 1/(1 + |c| + c^2 + a c^4)), cnnpp_cplusplus.cpp:662-666, with the
@@ -35,7 +51,7 @@ __device__ __forceinline__ float rtanh(float c)
     float sgn = 1.f;
     if (c < 0.f) sgn = -1.f;
     const float den = ((fabsf(c) + c * c) + a * c * c * c * c) + 1.f;
-    return (1.f - 1.f / den) * sgn;
+    return (1.f - (IEEE_DIV ? 1.f / den : recip_ge1(den))) * sgn;
 }
 
 __device__ __forceinline__ float lrelu_bn(float conv, const float* __restrict__ p, int m, int M)
@@ -251,6 +267,9 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
 {
     using namespace s1;
     __shared__ float s_in[AH * AW];
+    __shared__ uint32_t s_row[AH], s_px[AW];
+    __shared__ float s_fx[AW], s_cx[AW], s_fy[AH], s_cy[AH];
+    static_assert(AW + AH <= S1A_THREADS, "one table entry per thread");
     const int tid = threadIdx.x;
     // clear the per-chunk candidate counts that k_stage1 accumulates
     for (int c = blockIdx.x * S1A_THREADS + tid; c < nchunks; c += gridDim.x * S1A_THREADS) counts[c] = 0;
@@ -272,24 +291,91 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     const int x0 = 2 * X0, y0 = 2 * Y0;
 
     // level window (clamped at the level border; clamped samples only feed L1 positions
-    // whose responses lie outside the response map). Samples go in batches of 8 per
-    // thread so that their gray-image loads are in flight together.
+    // whose responses lie outside the response map). The resize's Q12 LUT is separable
+    // (ImageResizer::preprocessing, image_resize.cpp:46-122): a table per window column
+    // (source column, x weights) and per window row (source row, y weights) is built
+    // first, then every sample is a table lookup, its source pixels and the bilinear
+    // formula of level_px. Samples go in batches so that their loads are in flight
+    // together.
     {
-        constexpr int NS = AW * AH, PER = (NS + S1A_THREADS - 1) / S1A_THREADS, B = 8;
-#pragma unroll 1
-        for (int b0 = 0; b0 < PER; b0 += B) {
-            float v[B];
-#pragma unroll
-            for (int k = 0; k < B; ++k) {
-                const int i = min(tid + (b0 + k) * S1A_THREADS, NS - 1);
-                const int iy = i / AW, ix = i - iy * AW;
-                const int lx = min(x0 + ix, L.lw - 1), ly = min(y0 + iy, L.lh - 1);
-                v[k] = level_px(g, gp, gw, gh, L, lx, ly);
+        if (tid < AW) {
+            const int lx = min(x0 + tid, L.lw - 1);
+            uint32_t px = (uint32_t)lx;
+            float fx = 0.f, cx = 0.f;
+            if (L.mode != 2) {
+                const uint32_t x = (uint32_t)lx * L.sclx;
+                px = min(x >> 12, (uint32_t)(gw - 2));
+                fx = (float)(x - ((x >> 12) << 12));
+                cx = 4096.f - fx;
             }
+            s_px[tid] = px;
+            s_fx[tid] = fx;
+            s_cx[tid] = cx;
+        } else if (tid < AW + AH) {
+            const int iy = tid - AW;
+            const int ly = min(y0 + iy, L.lh - 1);
+            uint32_t py = (uint32_t)ly;
+            float fy = 0.f, cy = 0.f;
+            if (L.mode == 0) {
+                py = min(((uint32_t)ly * L.scly) >> 12, (uint32_t)(gh - 1));
+            } else if (L.mode == 1) {
+                const uint32_t y = (uint32_t)ly * L.scly;
+                py = y >> 12;
+                fy = (float)(y - (py << 12));
+                cy = 4096.f - fy;
+                fy = fy / 16777216.f;
+                cy = cy / 16777216.f;
+                py = min(py, (uint32_t)(gh - 1));
+                if (py + 1 >= (uint32_t)gh) py--;
+            }
+            s_row[iy] = py * (uint32_t)gp;
+            s_fy[iy] = fy;
+            s_cy[iy] = cy;
+        }
+        __syncthreads();
+        constexpr int NS = AW * AH, PER = (NS + S1A_THREADS - 1) / S1A_THREADS, B = 9;
+        static_assert(PER % B == 0, "whole batches");
+        if (L.mode == 1) {
+#pragma unroll 1
+            for (int b0 = 0; b0 < PER; b0 += B) {
+                float p[B][4];
+                int ixs[B], iys[B];
 #pragma unroll
-            for (int k = 0; k < B; ++k) {
-                const int i = tid + (b0 + k) * S1A_THREADS;
-                if (i < NS) s_in[i] = v[k];
+                for (int k = 0; k < B; ++k) {
+                    const int i = min(tid + (b0 + k) * S1A_THREADS, NS - 1);
+                    const int iy = i / AW, ix = i - iy * AW;
+                    const float* r0 = g + (s_row[iy] + s_px[ix]);
+                    const float* r1 = r0 + gp;
+                    p[k][0] = r0[0];
+                    p[k][1] = r0[1];
+                    p[k][2] = r1[0];
+                    p[k][3] = r1[1];
+                    ixs[k] = ix;
+                    iys[k] = iy;
+                }
+#pragma unroll
+                for (int k = 0; k < B; ++k) {
+                    const float cx = s_cx[ixs[k]], fx = s_fx[ixs[k]], cy = s_cy[iys[k]], fy = s_fy[iys[k]];
+                    const float v = (p[k][0] * cx + p[k][1] * fx) * cy + (p[k][2] * cx + p[k][3] * fx) * fy;
+                    const int i = tid + (b0 + k) * S1A_THREADS;
+                    if (i < NS) s_in[i] = v;
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int b0 = 0; b0 < PER; b0 += B) {
+                float v[B];
+#pragma unroll
+                for (int k = 0; k < B; ++k) {
+                    const int i = min(tid + (b0 + k) * S1A_THREADS, NS - 1);
+                    const int iy = i / AW, ix = i - iy * AW;
+                    v[k] = g[s_row[iy] + s_px[ix]];
+                }
+#pragma unroll
+                for (int k = 0; k < B; ++k) {
+                    const int i = tid + (b0 + k) * S1A_THREADS;
+                    if (i < NS) s_in[i] = v[k];
+                }
             }
         }
     }
@@ -304,7 +390,7 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     for (int p = tid; p < AX * AY; p += S1A_THREADS) {
         const int py = p / AX, px = p - py * AX;
         const int X = X0 + px, Y = Y0 + py;
-        if (X >= L.s1_w || Y >= L.s1_h) continue;
+        if (X >= 2 * L.rw + 8 || Y >= 2 * L.rh + 10) continue;
         float win[5][5];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
@@ -1179,6 +1265,30 @@ __global__ void __launch_bounds__(FA_THREADS) k_landmarks(const float* __restric
     if (threadIdx.x < Landmarks::NOUT) out[threadIdx.x] = s.out[threadIdx.x];
 }
 
+// Self-test of the exact-path reciprocal: over float bit patterns [lo, hi) (NaNs
+// skipped), counts the inputs where rtanh with recip_ge1 differs in any bit from rtanh
+// with IEEE division (out[0]) and, for inputs in [1, 2^25), where recip_ge1 differs from
+// 1/d (out[1]); out[2] keeps the largest mismatching pattern.
+__global__ void __launch_bounds__(256) k_selftest_tanh(uint32_t lo, uint32_t hi,
+                                                       unsigned long long* __restrict__ out)
+{
+    unsigned long long bad_t = 0, bad_r = 0, last = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t b = (uint64_t)lo + blockIdx.x * 256 + threadIdx.x; b < hi; b += stride) {
+        const float c = __uint_as_float((uint32_t)b);
+        if (c != c) continue;
+        const float t0 = rtanh<false>(c), t1 = rtanh<true>(c);
+        if (__float_as_uint(t0) != __float_as_uint(t1)) { ++bad_t; last = b; }
+        if (c >= 1.f && c < 33554432.f) {
+            const float r0 = recip_ge1(c), r1 = 1.f / c;
+            if (__float_as_uint(r0) != __float_as_uint(r1)) { ++bad_r; last = b; }
+        }
+    }
+    if (bad_t) atomicAdd(&out[0], bad_t);
+    if (bad_r) atomicAdd(&out[1], bad_r);
+    if (bad_t || bad_r) atomicMax(&out[2], last);
+}
+
 }  // namespace ccnn
 
 // ---------------------------------------------------------------------------------
@@ -1278,6 +1388,12 @@ hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelD
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st)
 {
     k_facial<<<nblocks, FA_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, W, results);
+    return hipGetLastError();
+}
+
+hipError_t launch_selftest_tanh(uint32_t lo, uint32_t hi, unsigned long long* out, hipStream_t st)
+{
+    k_selftest_tanh<<<8192, 256, 0, st>>>(lo, hi, out);
     return hipGetLastError();
 }
 

@@ -76,6 +76,18 @@ int ccnn_stage1_size(int lw, int lh, int* rw, int* rh)
     return (*rw > 0 && *rh > 0) ? 0 : -1;
 }
 
+int ccnn_selftest_tanh(unsigned int lo, unsigned int hi, unsigned long long* out)
+{
+    if (!out || hi < lo) return -1;
+    if (!have_device()) return -1;
+    DevBuf d;
+    if (!d.alloc(3 * sizeof(unsigned long long))) return -1;
+    if (hipMemset(d.p, 0, 3 * sizeof(unsigned long long)) != hipSuccess) return -1;
+    if (ccnn::launch_selftest_tanh(lo, hi, d.as<unsigned long long>(), nullptr) != hipSuccess) return -1;
+    if (hipMemcpy(out, d.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
 int ccnn_stage_landmarks(const float* in, float* out)
 {
     if (!in || !out) return -1;
@@ -120,7 +132,8 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out)
     if (!g.alloc(sizeof(float) * (size_t)lw * (lh + 1)) || !w.alloc(sizeof(float) * net.w.size()) ||
         !r.alloc(sizeof(float) * (size_t)rw * rh) || !l.alloc(sizeof(L)) || !s1.alloc(sizeof(float) * s1n))
         return -1;
-    if (hipMemcpy(g.p, level, sizeof(float) * (size_t)lw * lh, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemset(s1.p, 0, sizeof(float) * s1n) != hipSuccess ||
+        hipMemcpy(g.p, level, sizeof(float) * (size_t)lw * lh, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(w.p, net.w.data(), sizeof(float) * net.w.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(l.p, &L, sizeof(L), hipMemcpyHostToDevice) != hipSuccess)
         return -1;

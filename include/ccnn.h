@@ -164,6 +164,16 @@ int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
  * checks accepted and the landmarks rejected. Returns the candidate count; out may be NULL. */
 int ccnn_last_facial(const ccnn_detector* d, int* out, int cap);
 
+/* Self-test of the exact path's reciprocal (the rational tanh of the hidden and output
+ * layers, cnnpp_cplusplus.cpp:662-666, divides by a denominator >= 1; the kernels take
+ * that reciprocal from the hardware estimate and one fused Newton step instead of the
+ * general IEEE division). Over the float bit patterns [lo, hi) (NaNs skipped) on the
+ * GPU: out[0] counts inputs where the tanh differs in any bit from the one with IEEE
+ * division, out[1] inputs in [1, 2^25) where the reciprocal differs from 1/d (beyond
+ * 2^25, 1 - 1/d rounds to 1 and the kernels use 0), out[2] the largest mismatching
+ * pattern. No counterpart in the reference. 0, or -1. */
+int ccnn_selftest_tanh(unsigned int lo, unsigned int hi, unsigned long long* out);
+
 /* The last HIP error code the library met on the calling thread (0 if none), for callers
  * that need to tell "no faces" from a failed call (Detect reports 0 faces either way, as
  * the reference's does); reset != 0 clears it. */

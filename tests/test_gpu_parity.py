@@ -168,6 +168,20 @@ def test_trace_matches_oracle(api, oracle):
     assert bits(res[:, 4]).tolist() == bits(tr["c_m1"][ev]).tolist()
 
 
+def test_exact_reciprocal_matches_ieee_division_on_every_float(api):
+    # the hidden/output layers' rational tanh (cnnpp_cplusplus.cpp:662-666) divides by a
+    # denominator >= 1; the kernels take 1/den from the hardware estimate and one fused
+    # Newton step (0 beyond 2^25, where 1 - 1/d rounds to 1). Exhaustive over all 2^32
+    # bit patterns: the tanh is bit-identical to the one with IEEE division, and so is the
+    # reciprocal itself for every d in [1, 2^25).
+    import ctypes
+    L = api.load()
+    L.ccnn_selftest_tanh.argtypes = [ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_ulonglong)]
+    out = (ctypes.c_ulonglong * 3)()
+    assert L.ccnn_selftest_tanh(0, 0xFFFFFFFF, out) == 0
+    assert (out[0], out[1]) == (0, 0), "mismatches: tanh %d, reciprocal %d, last pattern %#x" % tuple(out)
+
+
 def test_landmarks_network_bitexact(api, oracle, vectors):
     rng = np.random.default_rng(12)
     inputs = list(vectors["fa_in"]) + [(rng.random((51, 49)) * 255).astype(np.float32) for _ in range(4)]
