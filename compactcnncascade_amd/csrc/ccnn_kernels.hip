@@ -48,10 +48,10 @@ __device__ __forceinline__ float rtanh(float c)
     // sgn * (1 - 1/(1 + |c| + c^2 + a c^4)), cnnpp_cplusplus.cpp:662-666, with the
     // association of the reference's -ffast-math build: ((|c| + c*c) + a*c*c*c*c) + 1.
     const float a = 1.41645f;
-    float sgn = 1.f;
-    if (c < 0.f) sgn = -1.f;
     const float den = ((fabsf(c) + c * c) + a * c * c * c * c) + 1.f;
-    return (1.f - (IEEE_DIV ? 1.f / den : recip_ge1(den))) * sgn;
+    const float t = 1.f - (IEEE_DIV ? 1.f / den : recip_ge1(den));
+    // the reference multiplies by sgn = -1 or 1, which is the negation or t itself
+    return c < 0.f ? -t : t;
 }
 
 __device__ __forceinline__ float lrelu_bn(float conv, const float* __restrict__ p, int m, int M)
@@ -426,10 +426,17 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 {
     using namespace s1;
     static_assert(TX * TY <= S1_THREADS, "one response value per thread");
-    __shared__ float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
-    __shared__ float s_p2[N::M2 * P2N];
+    __shared__ __align__(16) float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
+    __shared__ __align__(16) float s_p2[N::M2 * P2N];     // L2 maps; later the tail weights
+    // the L3, hidden- and output-layer weights, which the per-response tail reads from
+    // LDS; fetched into registers now, stored over the consumed L2 maps later
+    constexpr int WT = N::SIZE - N::oK3, WPER = (WT + S1_THREADS - 1) / S1_THREADS;
+    static_assert(WT <= N::M2 * P2N, "tail weights fit in the L2 map space");
 
     const int tid = threadIdx.x;
+    float wt[WPER];
+#pragma unroll
+    for (int k = 0; k < WPER; ++k) wt[k] = W[N::oK3 + min(tid + k * S1_THREADS, WT - 1)];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int li = find_level<BY_TILE>(levels, nlevels, tile);
     const LevelDesc L = levels[li];
@@ -437,26 +444,28 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
 
     // the tile's L1 sums: rows 2*oy0 .. +P1H, columns 2*ox0 .. +P1W of each plane
-    // (inside the planes: s1_w = 2*TX*tiles_x + 8, s1_h = 2*TY*tiles_y + 10)
+    // (inside the planes: s1_w = 2*TX*tiles_x + 8, s1_h = 2*TY*tiles_y + 10). Every
+    // plane row and level offset is a multiple of 8 floats and 2*ox0 of 32, so the rows
+    // are copied as float4; the M1 planes stack as M1*P1H rows of the tile window.
     {
-        const size_t plane = (size_t)L.s1_w * L.s1_h;
+        static_assert(P1W % 4 == 0 && (2 * TX) % 4 == 0, "float4 rows");
+        constexpr int RQ = P1W / 4, NQ = N::M1 * P1H * RQ;
+        constexpr int PER = (NQ + S1_THREADS - 1) / S1_THREADS, B = 7;
+        static_assert(PER <= B, "one batch");
+        const uint32_t plane = (uint32_t)L.s1_w * L.s1_h;
         const float* src = s1buf + L.s1_off + (size_t)(2 * oy0) * L.s1_w + 2 * ox0;
-        constexpr int NS = N::M1 * P1N, PER = (NS + S1_THREADS - 1) / S1_THREADS, B = 8;
-#pragma unroll 1
-        for (int b0 = 0; b0 < PER; b0 += B) {
-            float v[B];
+        float4 v[B];
 #pragma unroll
-            for (int k = 0; k < B; ++k) {
-                const int i = min(tid + (b0 + k) * S1_THREADS, NS - 1);
-                const int m = i / P1N, r = i - m * P1N;
-                const int py = r / P1W, px = r - py * P1W;
-                v[k] = src[m * plane + (size_t)py * L.s1_w + px];
-            }
+        for (int k = 0; k < B; ++k) {
+            const int i = min(tid + k * S1_THREADS, NQ - 1);
+            const int row = i / RQ, q = i - row * RQ;       // row of the stacked planes
+            const int m = row / P1H, py = row - m * P1H;
+            v[k] = *reinterpret_cast<const float4*>(src + m * plane + (uint32_t)py * L.s1_w + 4 * q);
+        }
 #pragma unroll
-            for (int k = 0; k < B; ++k) {
-                const int i = tid + (b0 + k) * S1_THREADS;
-                if (i < NS) s_s1[i] = v[k];
-            }
+        for (int k = 0; k < B; ++k) {
+            const int i = tid + k * S1_THREADS;
+            if (i < NQ) *reinterpret_cast<float4*>(s_s1 + 4 * i) = v[k];
         }
     }
     __syncthreads();
@@ -501,14 +510,20 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         s_s2[(N::M2 - 1) * P2N + p] = v[N::M2 - 2] + v[N::M2 - 1];
     }
     __syncthreads();
+    float* s_wt = s_p2;
+#pragma unroll
+    for (int k = 0; k < WPER; ++k)
+        if (tid + k * S1_THREADS < WT) s_wt[tid + k * S1_THREADS] = wt[k];
+    __syncthreads();
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread.
     // threads past the tile or outside the response map stop here (no barrier follows)
     const int ox = tid % TX, oy = tid / TX;
     if (tid >= TX * TY || ox0 + ox >= L.rw || oy0 + oy >= L.rh) return;
-    const float* k3 = W + N::oK3;
-    const float* b3 = W + N::oB3;
+    const float* Wt = s_wt - N::oK3;   // indexed with the offsets into W
+    const float* k3 = Wt + N::oK3;
+    const float* b3 = Wt + N::oB3;
     float a3[N::M3];
 #pragma unroll
     for (int i = 0; i < N::M2; ++i) {
@@ -540,13 +555,13 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 #pragma unroll
         for (int tt = 0; tt < N::HLS; ++tt) {
             const int u = N::HLS * gi + tt;
-            float c = W[N::oHB + u];
+            float c = Wt[N::oHB + u];
 #pragma unroll
-            for (int j = 0; j < N::HLN; ++j) c = c + a3[gi + j] * W[N::oHW + u * N::HLN + j];
-            c = c * W[N::oHT + gi];
+            for (int j = 0; j < N::HLN; ++j) c = c + a3[gi + j] * Wt[N::oHW + u * N::HLN + j];
+            c = c * Wt[N::oHT + gi];
             c = rtanh(c);
             c = (c + 1.f) * 0.5f;
-            hv[u] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+            hv[u] = Wt[N::oHBW + tt] * c + Wt[N::oHBB + tt];
         }
     // output layer: mulC / mulC1_add / tanhW (cnnpp_cplusplus.cpp:952-984, 673-690)
     float o = 0.f;
@@ -554,14 +569,14 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     for (int gi = 0; gi < N::HLG; ++gi)
 #pragma unroll
         for (int tt = 0; tt < N::HLS; ++tt) {
-            const float wv = W[N::oOW + tt * N::HLG + gi];
+            const float wv = Wt[N::oOW + tt * N::HLG + gi];
             if (gi == 0 && tt == 0) o = hv[0] * wv;
             else o = hv[N::HLS * gi + tt] * wv + o;
         }
-    float c = o + W[N::oOB];
-    c = c * W[N::oOB + 1];
+    float c = o + Wt[N::oOB];
+    c = c * Wt[N::oOB + 1];
     c = rtanh(c);
-    const float af = W[N::oOB + 2];
+    const float af = Wt[N::oOB + 2];
     const float out = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     const int e = (oy0 + oy) * L.rw + ox0 + ox;
     resp[L.resp_off + e] = out;
