@@ -667,17 +667,27 @@ constexpr int HLMAX = 84;
 constexpr int WL = Stage2::SIZE > Stage3::SIZE ? Stage2::SIZE : Stage3::SIZE;
 }  // namespace chk
 
+// LDS of one (candidate, crop variant). Regions are reused as their contents die:
+//   in  the network input; after L1 the L2 maps; after the L2 sums the L3 maps
+//       followed by the output
+//   s1  the crop's histogram and LUT while the crop is prepared; then the L1 sums;
+//       after L2 the L2 sums; after L3 the hidden layer
+//   w   the weights of the running network (stage 2 or 3)
 struct ChkSmem {
-    float in[chk::IH * chk::IW];         // stage-2 input, later the stage-3 input
-    float s1[6 * chk::P1N];
-    float p2[12 * chk::P2N];
-    float a3[24 * chk::RN];
-    float h[chk::HLMAX * chk::RN];
-    alignas(16) float out[chk::RN];
-    float w[chk::WL];                    // weights of the running network (stage 2 or 3)
-    int hist[256];
-    int lut[256];
+    alignas(16) float in[chk::IH * chk::IW];
+    alignas(16) float s1[6 * chk::P1N];
+    float w[chk::WL];
+    __device__ float* p2() { return in; }
+    __device__ float* a3() { return in; }
+    __device__ float* out() { return in + 24 * chk::RN; }
+    __device__ float* s2() { return s1; }
+    __device__ float* h() { return s1; }
+    __device__ int* hist() { return reinterpret_cast<int*>(s1); }
+    __device__ int* lut() { return reinterpret_cast<int*>(s1) + 256; }
 };
+static_assert(12 * chk::P2N <= chk::IH * chk::IW && 24 * chk::RN + chk::RN <= chk::IH * chk::IW &&
+                  12 * chk::P2N <= 6 * chk::P1N && chk::HLMAX * chk::RN <= 6 * chk::P1N && 512 <= 6 * chk::P1N,
+              "region reuse fits");
 
 template <class N>
 __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid)
@@ -738,15 +748,15 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
                 for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
                 a[q] = lrelu_bn(acc, Wl + N::oB2, m, N::M2);
             }
-            s.p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+            s.p2()[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
     }
     __syncthreads();
-    float* s2 = s.s1;
+    float* s2 = s.s2();
     for (int p = tid; p < P2N; p += CHECK_THREADS) {
         float v[N::M2];
 #pragma unroll
-        for (int m = 0; m < N::M2; ++m) v[m] = s.p2[m * P2N + p];
+        for (int m = 0; m < N::M2; ++m) v[m] = s.p2()[m * P2N + p];
         s2[p] = v[0] + v[1];
 #pragma unroll
         for (int i = 1; i < N::M2 - 1; ++i) s2[i * P2N + p] = v[i - 1] + v[i] + v[i + 1];
@@ -766,7 +776,7 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
 #pragma unroll
             for (int j = 1; j < N::K3W * N::K3H; ++j)
                 acc = acc + src[(j / N::K3W) * P2W + (j % N::K3W)] * k[j];
-            s.a3[m * RN + p] = lrelu_bn(acc, Wl + N::oB3, m, N::M3);
+            s.a3()[m * RN + p] = lrelu_bn(acc, Wl + N::oB3, m, N::M3);
         }
     }
     __syncthreads();
@@ -776,11 +786,11 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
         const int gi = u / N::HLS, tt = u - gi * N::HLS;
         float c = Wl[N::oHB + u];
 #pragma unroll
-        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[(gi + j) * RN + p] * Wl[N::oHW + u * N::HLN + j];
+        for (int j = 0; j < N::HLN; ++j) c = c + s.a3()[(gi + j) * RN + p] * Wl[N::oHW + u * N::HLN + j];
         c = c * Wl[N::oHT + gi];
         c = rtanh(c);
         c = (c + 1.f) * 0.5f;
-        s.h[u * RN + p] = Wl[N::oHBW + tt] * c + Wl[N::oHBB + tt];
+        s.h()[u * RN + p] = Wl[N::oHBW + tt] * c + Wl[N::oHBB + tt];
     }
     __syncthreads();
     // output layer
@@ -789,14 +799,14 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
         for (int gi = 0; gi < N::HLG; ++gi)
             for (int tt = 0; tt < N::HLS; ++tt) {
                 const float wv = Wl[N::oOW + tt * N::HLG + gi];
-                if (gi == 0 && tt == 0) o = s.h[tid] * wv;
-                else o = s.h[(N::HLS * gi + tt) * RN + tid] * wv + o;
+                if (gi == 0 && tt == 0) o = s.h()[tid] * wv;
+                else o = s.h()[(N::HLS * gi + tt) * RN + tid] * wv + o;
             }
         float c = o + Wl[N::oOB];
         c = c * Wl[N::oOB + 1];
         c = rtanh(c);
         const float af = Wl[N::oOB + 2];
-        s.out[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
+        s.out()[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     }
     __syncthreads();
 }
@@ -876,7 +886,7 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     const uint32_t sclx = ((uint32_t)cr.rcols << 12) / CW + 1;
     const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
     static_assert(CHECK_THREADS >= 256 && CHECK_THREADS % 64 == 0, "one histogram bin per thread");
-    if (tid < 256) s.hist[tid] = 0;
+    if (tid < 256) s.hist()[tid] = 0;
     __syncthreads();
     // all of a thread's gathers are issued before any is used, so that their latencies
     // overlap; 32-bit offsets from the crop's corner keep one register per address
@@ -897,18 +907,18 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
     if (equalize) {
 #pragma unroll
         for (int k = 0; k < PER; ++k)
-            if (tid + k * CHECK_THREADS < CW * CH) atomicAdd(&s.hist[u[k]], 1);
+            if (tid + k * CHECK_THREADS < CW * CH) atomicAdd(&s.hist()[u[k]], 1);
         __syncthreads();
-        const int cf = block_incl_scan(tid < 256 ? s.hist[tid] : 0, s_w);
-        if (tid < 256) s.lut[tid] = cf;
+        const int cf = block_incl_scan(tid < 256 ? s.hist()[tid] : 0, s_w);
+        if (tid < 256) s.lut()[tid] = cf;
         __syncthreads();
-        const float bmin = (float)s.lut[0], bmax = (float)s.lut[255];
+        const float bmin = (float)s.lut()[0], bmax = (float)s.lut()[255];
         remap = bmin != bmax;
         if (remap) {
             const float delt = 255.f / (bmax - bmin);
             const int v = (int)(((float)cf - bmin) * delt + 0.5f);
             __syncthreads();
-            if (tid < 256) s.lut[tid] = v;
+            if (tid < 256) s.lut()[tid] = v;
             __syncthreads();
         }
     }
@@ -917,7 +927,9 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
 // A network input from the thread's crop pixels: stage 2 reads UCharToFloat(offset 2)
 // (image_proc.cpp:416-427, crop columns 2..48), stage 3 UCharToFloat_inv (mirror,
 // :478-490, columns 46..0) or UCharToFloat(offset 0) (columns 0..46).
-__device__ void write_input(ChkSmem& s, const uint32_t (&u)[chk::PER], bool remap, bool stage3, bool reflection,
+// With remap, the pixels are first replaced by their equalized values (in u, for the
+// stage-3 input, as the LUT's LDS is reused by the network).
+__device__ void write_input(ChkSmem& s, uint32_t (&u)[chk::PER], bool remap, bool stage3, bool reflection,
                             const int tid)
 {
     using namespace chk;
@@ -926,7 +938,8 @@ __device__ void write_input(ChkSmem& s, const uint32_t (&u)[chk::PER], bool rema
         const int i = tid + k * CHECK_THREADS;
         if (i < CW * CH) {
             const int iy = i / CW, ix = i - iy * CW;
-            const float f = (float)(remap ? (uint32_t)s.lut[u[k]] & 255u : u[k]);
+            if (remap) u[k] = (uint32_t)s.lut()[u[k]] & 255u;
+            const float f = (float)u[k];
             if (!stage3) {
                 if (ix >= 2) s.in[iy * IW + ix - 2] = f;
             } else if (ix <= 46) {
@@ -943,7 +956,7 @@ __device__ void write_input(ChkSmem& s, const uint32_t (&u)[chk::PER], bool rema
 // the stage-3 network on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0.
 // s2res / s3res[2*ci + mod] = {count above the threshold, max response bits};
 // s3res {-1, -10.f} when stage 3 is skipped.
-__global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict__ g, int gp, int gw, int gh,
+__global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restrict__ g, int gp, int gw, int gh,
                                                          const LevelDesc* __restrict__ levels,
                                                          const Candidate* __restrict__ cands,
                                                          const int* __restrict__ meta, CheckParams P,
@@ -981,7 +994,7 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
             float d[RN];
 #pragma unroll
             for (int q = 0; q < RN / 4; ++q) {
-                const float4 v = reinterpret_cast<const float4*>(s.out)[q];
+                const float4 v = reinterpret_cast<const float4*>(s.out())[q];
                 d[4 * q] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
             }
             float m1 = -10.f;
@@ -1006,13 +1019,13 @@ __global__ void __launch_bounds__(CHECK_THREADS) k_check(const float* __restrict
         }
         // stage-3 weights; the barrier closing write_input orders them
         stage_weights<Stage3::SIZE>(s.w, W3, tid);
-        write_input(s, u, remap, true, P.reflection != 0, tid);
+        write_input(s, u, false, true, P.reflection != 0, tid);
         check_forward<Stage3>(s.in, s, tid);
         if (tid == 0) {
             float d[RN];
 #pragma unroll
             for (int q = 0; q < RN / 4; ++q) {
-                const float4 v = reinterpret_cast<const float4*>(s.out)[q];
+                const float4 v = reinterpret_cast<const float4*>(s.out())[q];
                 d[4 * q] = v.x; d[4 * q + 1] = v.y; d[4 * q + 2] = v.z; d[4 * q + 3] = v.w;
             }
             float m2 = -10.f;
