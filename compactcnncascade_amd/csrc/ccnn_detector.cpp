@@ -406,7 +406,7 @@ void Detector::release_slot(Slot& s)
     dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_s1); dfree(s.d_levels);
     dfree(s.d_l1_order);
     dfree(s.d_counts);
-    dfree(s.d_offsets); dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
+    dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
     hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
     for (auto& e : s.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
@@ -626,7 +626,7 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
         s.s1_cap = plan.s1_total;
     }
     if ((size_t)plan.nchunks > s.chunk_cap) {
-        if (dalloc(s.d_counts, plan.nchunks) < 0 || dalloc(s.d_offsets, plan.nchunks) < 0) return -1;
+        if (dalloc(s.d_counts, plan.nchunks) < 0) return -1;
         s.chunk_cap = plan.nchunks;
     }
     const size_t hc = std::min(std::max<size_t>(plan.resp_total, PREFETCH), PINNED_CAP);
@@ -722,7 +722,7 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                          profiling_ ? s.kev : nullptr));
     if (profiling_ && plan->ntiles <= 0)
         for (int k = 0; k < 4; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
-    HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_offsets, s.d_meta,
+    HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_meta,
                           s.d_cands, st));
     if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;

@@ -13,7 +13,7 @@
 //   k_stage1       per 16x16 response tile: L2, L3, hidden   (cnn_simd_cntk.cpp:726-963)
 //                  and output layers; counts the candidate
 //                  windows (score > t1) per compaction chunk
-//   k_scan / k_scatter
+//   k_scatter
 //                  order-preserving compaction of the        (cnn_detector_v3.cpp:1918-1947)
 //                  candidate windows (levels in check order, raster order inside a level)
 //   k_check        per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
@@ -589,37 +589,22 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 // Candidate compaction: responses > t1, levels in check order (the level table is in
 // that order), raster order inside a level.
 // ---------------------------------------------------------------------------------
-// Single workgroup: exclusive scan over the chunk counts; total written to meta[0].
-__global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, int nchunks, int* __restrict__ offsets,
-                                               int* __restrict__ meta)
-{
-    __shared__ int s_tmp[1024];
-    const int tid = threadIdx.x;
-    const int per = (nchunks + 1023) / 1024;
-    const int b = tid * per;
-    int sum = 0;
-    for (int k = 0; k < per; ++k)
-        if (b + k < nchunks) sum += counts[b + k];
-    s_tmp[tid] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int a = tid >= off ? s_tmp[tid - off] : 0;
-        __syncthreads();
-        s_tmp[tid] += a;
-        __syncthreads();
-    }
-    int run = s_tmp[tid] - sum;
-    for (int k = 0; k < per; ++k)
-        if (b + k < nchunks) { offsets[b + k] = run; run += counts[b + k]; }
-    if (tid == 1023) meta[0] = s_tmp[1023];
-}
-
+// One workgroup per chunk of CMP_CHUNK responses: the chunk's first output slot is the
+// sum of the counts of the chunks before it (read whole by every workgroup: a few
+// hundred to a few thousand ints); the last chunk's workgroup writes the total to meta[0].
 __global__ void __launch_bounds__(256) k_scatter(const float* __restrict__ resp, const LevelDesc* __restrict__ levels,
-                                                 int nlevels, float t1, const int* __restrict__ offsets,
-                                                 Candidate* __restrict__ cands)
+                                                 int nlevels, float t1, const int* __restrict__ counts,
+                                                 Candidate* __restrict__ cands, int* __restrict__ meta)
 {
     __shared__ int s_tmp[256];
     const int c = blockIdx.x;
+    int before = 0;
+    for (int i = threadIdx.x; i < c; i += 256) before += counts[i];
+    {
+        int tot;
+        block_excl_scan_256(before, s_tmp, &tot);
+        before = tot;
+    }
     const int li = find_level<BY_CHUNK>(levels, nlevels, c);
     const LevelDesc L = levels[li];
     const int n = L.rw * L.rh;
@@ -634,7 +619,8 @@ __global__ void __launch_bounds__(256) k_scatter(const float* __restrict__ resp,
         if (e < n && v[k] > t1) cnt++;
     }
     int total;
-    int pos = offsets[c] + block_excl_scan_256(cnt, s_tmp, &total);
+    int pos = before + block_excl_scan_256(cnt, s_tmp, &total);
+    if (c == gridDim.x - 1 && threadIdx.x == 0) meta[0] = before + total;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int e = base + threadIdx.x * PER + k;
@@ -1380,12 +1366,11 @@ hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelD
 }
 
 hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
-                          int* counts, int* offsets, int* meta, Candidate* cands, hipStream_t st)
+                          int* counts, int* meta, Candidate* cands, hipStream_t st)
 {
     // counts: per-chunk candidate counts accumulated by k_stage1
     if (nchunks <= 0) return hipMemsetAsync(meta, 0, sizeof(int), st);
-    k_scan<<<1, 1024, 0, st>>>(counts, nchunks, offsets, meta);
-    k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, offsets, cands);
+    k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, counts, cands, meta);
     return hipGetLastError();
 }
 
