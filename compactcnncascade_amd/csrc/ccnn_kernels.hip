@@ -387,10 +387,14 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     const float* b1 = W + N::oB1;
     const size_t plane = (size_t)L.s1_w * L.s1_h;
     float* out = s1buf + L.s1_off;
-    for (int p = tid; p < AX * AY; p += S1A_THREADS) {
-        const int py = p / AX, px = p - py * AX;
+    // the positions responses inside the map read (vx x vy of the tile; fewer at the
+    // level's right and bottom edges), numbered densely so that edge tiles issue fewer
+    // waves rather than masked lanes (p / vx from a float reciprocal: exact here)
+    const int vx = min(AX, 2 * L.rw + 8 - X0), vy = min(AY, 2 * L.rh + 10 - Y0);
+    const float rvx = 1.f / (float)vx;
+    for (int p = tid; p < vx * vy; p += S1A_THREADS) {
+        const int py = (int)(((float)p + 0.5f) * rvx), px = p - py * vx;
         const int X = X0 + px, Y = Y0 + py;
-        if (X >= 2 * L.rw + 8 || Y >= 2 * L.rh + 10) continue;
         float win[5][5];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
@@ -442,6 +446,14 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const LevelDesc L = levels[li];
     const int t = tile - L.tile_off;
     const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
+    // the tile's responses inside the map (vw x vh; fewer at the map's right and bottom
+    // edges) and the pooled L2 positions they read (qw x qh). Work items are numbered
+    // over these only, so that edge tiles issue fewer waves rather than masked lanes;
+    // small quotients come from float reciprocals (exact for these ranges: the margins
+    // to the next integer are far above float rounding).
+    const int vw = min(TX, L.rw - ox0), vh = min(TY, L.rh - oy0);
+    const int qw = vw + 3, qh = vh + 4, nq = qw * qh;
+    const float rqw = 1.f / (float)qw, rnq = 1.f / (float)nq, rvw = 1.f / (float)vw;
 
     // the tile's L1 sums: rows 2*oy0 .. +P1H, columns 2*ox0 .. +P1W of each plane
     // (inside the planes: s1_w = 2*TX*tiles_x + 8, s1_h = 2*TY*tiles_y + 10). Every
@@ -473,9 +485,10 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     // L2: two conv_3x3 chains per sum + lrelu_bn_max
     const float* k2 = W + N::oK2;
     const float* b2 = W + N::oB2;
-    for (int it = tid; it < N::M1 * P2N; it += S1_THREADS) {
-        const int i = it / P2N, p = it - i * P2N;
-        const int qy = p / P2W, qx = p - qy * P2W;
+    for (int it = tid; it < N::M1 * nq; it += S1_THREADS) {
+        const int i = (int)(((float)it + 0.5f) * rnq), r = it - i * nq;
+        const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
+        const int p = qy * P2W + qx;
         const float* src = s_s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
         float win[4][4];
 #pragma unroll
@@ -500,7 +513,9 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     }
     __syncthreads();
     float* s_s2 = s_s1;   // L2 sums overwrite the (consumed) L1 sums
-    for (int p = tid; p < P2N; p += S1_THREADS) {
+    for (int r = tid; r < nq; r += S1_THREADS) {
+        const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
+        const int p = qy * P2W + qx;
         float v[N::M2];
 #pragma unroll
         for (int m = 0; m < N::M2; ++m) v[m] = s_p2[m * P2N + p];
@@ -518,9 +533,9 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread.
-    // threads past the tile or outside the response map stop here (no barrier follows)
-    const int ox = tid % TX, oy = tid / TX;
-    if (tid >= TX * TY || ox0 + ox >= L.rw || oy0 + oy >= L.rh) return;
+    // threads past the responses inside the map stop here (no barrier follows)
+    if (tid >= vw * vh) return;
+    const int oy = (int)(((float)tid + 0.5f) * rvw), ox = tid - oy * vw;
     const float* Wt = s_wt - N::oK3;   // indexed with the offsets into W
     const float* k3 = Wt + N::oK3;
     const float* b3 = Wt + N::oB3;
