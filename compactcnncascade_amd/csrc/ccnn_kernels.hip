@@ -851,24 +851,6 @@ __device__ __forceinline__ Crop crop_geometry(int px, int py, float scale, int m
     return c;
 }
 
-// Inclusive scan of one int per thread over the workgroup (waves of 64; s_w holds one
-// int per wave).
-__device__ __forceinline__ int block_incl_scan(int v, int* s_w)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int a = __shfl_up(v, off, 64);
-        if (lane >= off) v += a;
-    }
-    if (lane == 63) s_w[w] = v;
-    __syncthreads();
-    int base = 0;
-    for (int k = 0; k < w; ++k) base += s_w[k];
-    __syncthreads();
-    return v + base;
-}
-
 // Crop of one (candidate, crop variant): NN resize to 49x51 (a copy at that size,
 // image_resize.cpp:328-392, 564-584), FloatToUChar (+0.5, truncate, image_proc.cpp:
 // 357-368), then equalizeImage (image_proc.cpp:720-800: cumulative histogram and LUT
@@ -902,26 +884,40 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
             py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
         }
         const float v = base[py * (uint32_t)gp + px];
-        u[k] = (uint32_t)(v + 0.5f) & 255u;
+        // the pixel value in bits 0-7, its crop column and row above, for write_input
+        u[k] = ((uint32_t)(v + 0.5f) & 255u) | ((uint32_t)ix << 8) | ((uint32_t)iy << 16);
     }
     remap = false;
     if (equalize) {
 #pragma unroll
         for (int k = 0; k < PER; ++k)
-            if (tid + k * CHECK_THREADS < CW * CH) atomicAdd(&s.hist()[u[k]], 1);
+            if (tid + k * CHECK_THREADS < CW * CH) atomicAdd(&s.hist()[u[k] & 255u], 1);
         __syncthreads();
-        const int cf = block_incl_scan(tid < 256 ? s.hist()[tid] : 0, s_w);
-        if (tid < 256) s.lut()[tid] = cf;
-        __syncthreads();
-        const float bmin = (float)s.lut()[0], bmax = (float)s.lut()[255];
-        remap = bmin != bmax;
-        if (remap) {
-            const float delt = 255.f / (bmax - bmin);
-            const int v = (int)(((float)cf - bmin) * delt + 0.5f);
-            __syncthreads();
-            if (tid < 256) s.lut()[tid] = v;
-            __syncthreads();
+        // cumulative histogram and LUT by the first wave, four bins per lane
+        if (tid < 64) {
+            const int4 h = reinterpret_cast<const int4*>(s.hist())[tid];
+            const int c0 = h.x, c1 = c0 + h.y, c2 = c1 + h.z, c3 = c2 + h.w;
+            int v = c3;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int a = __shfl_up(v, off, 64);
+                if (tid >= off) v += a;
+            }
+            const int b0 = v - c3;
+            int4 cf = make_int4(b0 + c0, b0 + c1, b0 + c2, b0 + c3);
+            const float bmin = (float)__shfl(cf.x, 0, 64), bmax = (float)__shfl(cf.w, 63, 64);
+            if (bmin != bmax) {
+                const float delt = 255.f / (bmax - bmin);
+                cf.x = (int)(((float)cf.x - bmin) * delt + 0.5f);
+                cf.y = (int)(((float)cf.y - bmin) * delt + 0.5f);
+                cf.z = (int)(((float)cf.z - bmin) * delt + 0.5f);
+                cf.w = (int)(((float)cf.w - bmin) * delt + 0.5f);
+            }
+            reinterpret_cast<int4*>(s.lut())[tid] = cf;
+            if (tid == 0) s_w[0] = bmin != bmax;
         }
+        __syncthreads();
+        remap = s_w[0] != 0;
     }
 }
 
@@ -934,13 +930,17 @@ __device__ void write_input(ChkSmem& s, uint32_t (&u)[chk::PER], bool remap, boo
                             const int tid)
 {
     using namespace chk;
+    if (remap) {
+        // every u[k] holds a crop pixel (the gather clamps the index), so all are mapped
+#pragma unroll
+        for (int k = 0; k < PER; ++k) u[k] = (u[k] & ~255u) | ((uint32_t)s.lut()[u[k] & 255u] & 255u);
+    }
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = tid + k * CHECK_THREADS;
         if (i < CW * CH) {
-            const int iy = i / CW, ix = i - iy * CW;
-            if (remap) u[k] = (uint32_t)s.lut()[u[k]] & 255u;
-            const float f = (float)u[k];
+            const int ix = (u[k] >> 8) & 255u, iy = u[k] >> 16;
+            const float f = (float)(u[k] & 255u);
             if (!stage3) {
                 if (ix >= 2) s.in[iy * IW + ix - 2] = f;
             } else if (ix <= 46) {
