@@ -48,6 +48,7 @@ CASES = {
     "vga_s11_ultra": (lambda: vga(11, smin=20), dict(min_face=20, precision=4)),
     "vga_s11_def": (lambda: vga(11, smin=20), dict(min_face=20, precision=0)),
     "vga_s11_disable": (lambda: vga(11, smin=20), dict(min_face=40, detect_mode=0)),
+    "vga_s11_sync": (lambda: vga(11, smin=20), dict(min_face=20, detect_mode=1)),
     "vga_s11_noeq_norefl_nodrop": (lambda: vga(11, smin=20), dict(min_face=20, equalize=0, reflection=0,
                                                                   drop_detect=0)),
     "vga_s11_bgra_min30": (lambda: _bgra(vga(11, smin=20)), dict(min_face=30)),
