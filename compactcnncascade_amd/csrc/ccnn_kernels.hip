@@ -697,9 +697,13 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
     // the network's weights, staged in LDS (s.w); read the same for every thread in L1,
     // per thread from L2 on
     const float* Wl = s.w;
-    // L1 + sums
-    for (int p = tid; p < P1N; p += CHECK_THREADS) {
-        const int py = p / P1W, px = p - py * P1W;
+    // L1 + sums: the first FULL positions one per lane (whole waves), the REM others by
+    // one wave with a lane per 2x2-pool quad. Index arithmetic is unsigned throughout.
+    const unsigned ut = (unsigned)tid;
+    constexpr unsigned FULL = P1N / 64 * 64, REM = P1N - FULL;
+    static_assert(REM == 0 || REM * 4 == 64, "the remainder fills one wave, four lanes a position");
+    for (unsigned p = ut; p < FULL; p += CHECK_THREADS) {
+        const unsigned py = p / P1W, px = p - py * P1W;
         float win[5][5];
 #pragma unroll
         for (int r = 0; r < 5; ++r)
@@ -725,11 +729,39 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
         for (int i = 1; i < N::M1 - 1; ++i) s.s1[i * P1N + p] = pm[i - 1] + pm[i] + pm[i + 1];
         s.s1[(N::M1 - 1) * P1N + p] = pm[N::M1 - 2] + pm[N::M1 - 1];
     }
+    if (REM > 0 && ut < REM * 4) {
+        const unsigned p = FULL + (ut >> 2), q = ut & 3u;
+        const unsigned py = p / P1W, px = p - py * P1W;
+        const unsigned dx = q & 1u, dy = q >> 1;
+        float win[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) win[r][c] = in[(2 * py + dy + r) * IW + 2 * px + dx + c];
+        float pm[N::M1];
+#pragma unroll
+        for (int m = 0; m < N::M1; ++m) {
+            const float* k = Wl + N::oK1 + m * 16;
+            float acc = win[0][0] * k[0];
+#pragma unroll
+            for (int j = 1; j < 16; ++j) acc = acc + win[j >> 2][j & 3] * k[j];
+            const float a = lrelu_bn(acc, Wl + N::oB1, m, N::M1);
+            // fmaxf(fmaxf(a0, a1), fmaxf(a2, a3)) across the position's four lanes
+            const float m01 = fmaxf(a, __shfl_xor(a, 1, 64));
+            pm[m] = fmaxf(m01, __shfl_xor(m01, 2, 64));
+        }
+        if (q == 0) {
+            s.s1[p] = pm[0] + pm[1];
+#pragma unroll
+            for (int i = 1; i < N::M1 - 1; ++i) s.s1[i * P1N + p] = pm[i - 1] + pm[i] + pm[i + 1];
+            s.s1[(N::M1 - 1) * P1N + p] = pm[N::M1 - 2] + pm[N::M1 - 1];
+        }
+    }
     __syncthreads();
     // L2
-    for (int it = tid; it < N::M1 * P2N; it += CHECK_THREADS) {
-        const int i = it / P2N, p = it - i * P2N;
-        const int qy = p / P2W, qx = p - qy * P2W;
+    for (unsigned it = ut; it < N::M1 * P2N; it += CHECK_THREADS) {
+        const unsigned i = it / P2N, p = it - i * P2N;
+        const unsigned qy = p / P2W, qx = p - qy * P2W;
         const float* src = s.s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
         float win[4][4];
 #pragma unroll
@@ -754,7 +786,7 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
     }
     __syncthreads();
     float* s2 = s.s2();
-    for (int p = tid; p < P2N; p += CHECK_THREADS) {
+    for (unsigned p = ut; p < P2N; p += CHECK_THREADS) {
         float v[N::M2];
 #pragma unroll
         for (int m = 0; m < N::M2; ++m) v[m] = s.p2()[m * P2N + p];
@@ -765,9 +797,9 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
     }
     __syncthreads();
     // L3: conv_8x7 single chain (cnnpp_cplusplus.cpp:292-321) + lrelu_bn
-    for (int it = tid; it < N::M2 * RN; it += CHECK_THREADS) {
-        const int i = it / RN, p = it - i * RN;
-        const int y = p / RW, x = p - y * RW;
+    for (unsigned it = ut; it < N::M2 * RN; it += CHECK_THREADS) {
+        const unsigned i = it / RN, p = it - i * RN;
+        const unsigned y = p / RW, x = p - y * RW;
         const float* src = s2 + i * P2N + y * P2W + x;
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -782,9 +814,9 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
     }
     __syncthreads();
     // hidden layer
-    for (int it = tid; it < N::HL * RN; it += CHECK_THREADS) {
-        const int u = it / RN, p = it - u * RN;
-        const int gi = u / N::HLS, tt = u - gi * N::HLS;
+    for (unsigned it = ut; it < N::HL * RN; it += CHECK_THREADS) {
+        const unsigned u = it / RN, p = it - u * RN;
+        const unsigned gi = u / N::HLS, tt = u - gi * N::HLS;
         float c = Wl[N::oHB + u];
 #pragma unroll
         for (int j = 0; j < N::HLN; ++j) c = c + s.a3()[(gi + j) * RN + p] * Wl[N::oHW + u * N::HLN + j];
