@@ -20,6 +20,11 @@
 //                  resize to 49x51, u8, equalize, stage 2,    image_proc.cpp:357-368,720-800)
 //                  stage 3 on the mirrored crop when needed
 //   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1
+//   k_resize_u8    8-bit bilinear pre-resize of 1-channel frames (image_resize.cpp:197-326)
+//   k_facial       landmarks network on the accepted windows     (cnn_detector_v3.cpp:1731-1754)
+//   k_landmarks    the landmarks network on one input (tests)
+//   k_export       read-back of candidates and results into mapped host memory
+//   k_selftest_tanh  the exact reciprocal against IEEE division over a range of floats
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
