@@ -223,6 +223,27 @@ def test_repeated_detect_is_stable(api):
         assert np.array_equal(first, again)
 
 
+def test_detect_capacity_truncates_but_reports_count(api):
+    # ccnn_detect writes min(count, capacity) faces and returns the full count; slots past
+    # the capacity stay untouched
+    import ctypes
+    from cases import vga
+    f = vga(11, smin=20)
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, dict(min_face=20))) == 0
+    n, full = d.Detect(f)
+    assert n >= 2
+    img, ci = api._image(f)
+    L = api.load()
+    for cap in (0, 1, n - 1):
+        out = np.zeros(n + 1, api.FACE_DTYPE)
+        out["x"] = -7
+        m = L.ccnn_detect(d._h, api._ptr(out), cap, ctypes.byref(ci))
+        assert m == n
+        assert np.array_equal(out[:cap], full[:cap])
+        assert (out["x"][cap:] == -7).all()
+
+
 def test_detect_on_row_strided_view(api):
     # ImageData with a row step wider than cols * channels (a crop of a bigger frame)
     from cases import vga
