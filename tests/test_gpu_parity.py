@@ -140,6 +140,46 @@ def test_detect_matches_reference(api, golden, name):
     assert [[int(r["gender"]), int(r["smile"]), int(r["glasses"])] for r in faces] == [e[5:8] for e in exp]
 
 
+def random_case(seed):
+    """A seeded random frame (size, channels, faces) and parameter set, over the
+    parameters FaceDetector::Param exposes."""
+    from cases import _bgra, _gray
+    from compactcnncascade_amd import synth
+    rng = np.random.default_rng(seed)
+    w, h = int(rng.integers(64, 561)), int(rng.integers(64, 421))
+    f = synth.synth_frame(1000 + seed, w, h, float(rng.uniform(2, 8)), 20, 160)[0]
+    ch = int(rng.choice([1, 3, 3, 4]))
+    f = _gray(f) if ch == 1 else _bgra(f) if ch == 4 else f
+    min_face = int(rng.integers(20, 61))
+    over = dict(min_face=min_face, max_face=int(rng.choice([0, min_face + int(rng.integers(10, 150))])),
+                scale_factor=float(rng.choice([1.05, 1.1, 1.15, 1.2, 1.3])), precision=int(rng.integers(0, 5)),
+                min_neighbors=int(rng.integers(1, 4)), equalize=int(rng.integers(0, 2)),
+                reflection=int(rng.integers(0, 2)), drop_detect=int(rng.integers(0, 2)),
+                detect_mode=int(rng.choice([0, 1, 2, 2])), t1=float(rng.uniform(-2.0, -1.0)),
+                t2=float(rng.uniform(-1.5, -0.75)), t3=float(rng.uniform(-1.25, -0.5)),
+                facial_analysis=int(rng.integers(0, 4) == 0))
+    return f, over
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_frames_and_parameters_match_oracle(api, oracle, seed):
+    # differential test over random frame sizes, channel counts and parameter sets: the
+    # GPU path against the oracle computed on the CPU for the same inputs (boxes and
+    # score bits exact, facial attributes equal)
+    f, over = random_case(seed)
+    print("random case", seed, f.shape, over, flush=True)
+    exp, _ = oracle.detect(f, params_for(f, over))
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    n, faces = d.Detect(f)
+    assert n == len(exp)
+    got = [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces]
+    assert got == [[int(v) for v in e[:4]] for e in exp]
+    assert bits(faces["score"]).tolist() == bits(np.asarray(exp[:, 4], np.float32)).tolist()
+    assert [[int(r["gender"]), int(r["smile"]), int(r["glasses"])] for r in faces] == \
+        [[int(v) for v in e[5:8]] for e in exp]
+
+
 def test_trace_matches_oracle(api, oracle):
     from cases import photo3
     f = photo3()
