@@ -161,7 +161,11 @@ def random_case(seed):
     return f, over
 
 
-@pytest.mark.parametrize("seed", range(16))
+# seeds of the randomized differential test; CCNN_RANDOM_CASES widens the sweep
+RANDOM_CASES = int(os.environ.get("CCNN_RANDOM_CASES", "16"))
+
+
+@pytest.mark.parametrize("seed", range(RANDOM_CASES))
 def test_random_frames_and_parameters_match_oracle(api, oracle, seed):
     # differential test over random frame sizes, channel counts and parameter sets: the
     # GPU path against the oracle computed on the CPU for the same inputs (boxes and
