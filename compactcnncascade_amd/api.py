@@ -218,9 +218,14 @@ class FaceDetector:
     def DetectDevice(self, dev_ptr, cols, rows, channels, step):
         """Detection on a frame already resident in GPU memory (device pointer)."""
         ci = CImage(cols, rows, channels, dev_ptr, step)
-        faces = np.zeros(4096, FACE_DTYPE)
-        n = self._L.ccnn_detect_device(self._h, _ptr(faces), 4096, ctypes.byref(ci))
-        return n, faces[:max(0, min(n, 4096))]
+        cap = 4096
+        while True:
+            faces = np.zeros(cap, FACE_DTYPE)
+            n = self._L.ccnn_detect_device(self._h, _ptr(faces), cap, ctypes.byref(ci))
+            if n <= cap:
+                break
+            cap = n              # more faces than the buffer holds: detect again into a bigger one
+        return n, faces[:max(0, n)]
 
     def DetectBatch(self, images=None, device_frames=None, capacity=256):
         """Pipelined detection of several frames. images: list of host arrays, or

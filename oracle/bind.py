@@ -190,6 +190,12 @@ class Oracle:
                 setattr(tr, k, _ptr(a))
         n = self.lib.oc_detect(*(ctypes.byref(m) for m in self.models), ctypes.byref(p), _ptr(img), w, h, ch,
                                w * ch, _ptr(raw), cap, ctypes.byref(tr) if tr is not None else None)
+        if n > cap:
+            # more faces than the buffer holds (raw boxes without Merger): run again
+            cap = n
+            raw = np.zeros(cap * 8, np.int32)
+            n = self.lib.oc_detect(*(ctypes.byref(m) for m in self.models), ctypes.byref(p), _ptr(img), w, h, ch,
+                                   w * ch, _ptr(raw), cap, ctypes.byref(tr) if tr is not None else None)
         if n < 0:
             return n, None
         rec = raw[:8 * n].reshape(n, 8)
