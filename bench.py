@@ -299,6 +299,10 @@ def main():
         # the metric's 4K leg and the mixed-size batch (not part of `value`)
         line["uhd_3840x2160"] = secondary_fps(api, torch, ["mixed_uhd"], 8, 3)
         line["mixed_batch"] = secondary_fps(api, torch, ["mixed_vga", "mixed_hd", "mixed_fhd", "mixed_uhd"], 8, 3)
+        # the reference's CPU-runnable case (640x480, faces >= 40) and the deep pyramid
+        # (4800x2400, scale 1.05, faces >= 15: 91 levels)
+        line["vga_640x480"] = secondary_fps(api, torch, ["vga_min40"], 64, 3)
+        line["deep_4800x2400"] = secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(frames, dict(min_face=cfg["min_face"], max_face=cfg["max_face"],
                                        scale_factor=cfg["scale_factor"]), args.cpu_seconds)
