@@ -405,7 +405,7 @@ void Detector::release_slot(Slot& s)
     if (s.st) (void)hipStreamSynchronize(s.st);
     dfree(s.d_img); dfree(s.d_pre); dfree(s.d_gray); dfree(s.d_resp); dfree(s.d_s1); dfree(s.d_levels);
     dfree(s.d_l1_order);
-    dfree(s.d_counts);
+
     dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
     hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
     for (auto& e : s.ev)
@@ -625,10 +625,6 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
         if (hipMemset(s.d_s1, 0, plan.s1_total * sizeof(float)) != hipSuccess) return -1;
         s.s1_cap = plan.s1_total;
     }
-    if ((size_t)plan.nchunks > s.chunk_cap) {
-        if (dalloc(s.d_counts, plan.nchunks) < 0) return -1;
-        s.chunk_cap = plan.nchunks;
-    }
     const size_t hc = std::min(std::max<size_t>(plan.resp_total, PREFETCH), PINNED_CAP);
     if (hc > s.h_cap) {
         if (halloc(s.h_cands, s.hd_cands, hc) < 0 || halloc(s.h_results, s.hd_results, hc) < 0) return -1;
@@ -717,13 +713,14 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
         s.levels_plan = plan;
     }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
+    // k_stage1_l1 zeroes the candidate count and k_stage1 appends the candidates
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
-                         s.d_resp, set_.t1, s.d_counts, plan->nchunks, s.d_l1_order, st,
-                         profiling_ ? s.kev : nullptr));
-    if (profiling_ && plan->ntiles <= 0)
-        for (int k = 0; k < 4; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
-    HIP_OK(launch_compact(s.d_resp, s.d_levels, nl, plan->nchunks, set_.t1, s.d_counts, s.d_meta,
-                          s.d_cands, st));
+                         s.d_resp, set_.t1, s.d_meta, s.d_cands, s.d_l1_order, st, profiling_ ? s.kev : nullptr));
+    if (plan->ntiles <= 0) {
+        HIP_OK(hipMemsetAsync(s.d_meta, 0, sizeof(int), st));
+        if (profiling_)
+            for (int k = 0; k < 4; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
+    }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
     // candidates and check results go straight to the mapped host buffers (up to s.pre):
@@ -792,11 +789,41 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         for (int i = 0; i < NT; ++i) acc_ms_[i] += last_ms_[i];
         acc_frames_++;
     }
+    // k_stage1 appends the candidate windows in no particular order; the reference visits
+    // them level by level (check order) and in raster order inside a level. Bucket them
+    // by compaction chunk (CMP_CHUNK consecutive raster positions of one level, chunks in
+    // check order), then order each bucket by raster position.
+    {
+        const int nchunks = s.plan->nchunks;
+        std::vector<int> start(nchunks + 2, 0), ord(count);
+        std::vector<int> key(count);
+        for (int i = 0; i < count; ++i) {
+            const LevelDesc& L = levels[cands[i].level];
+            const int e = cands[i].y * L.rw + cands[i].x;
+            key[i] = e;
+            start[L.chunk_off + e / CMP_CHUNK + 2]++;
+        }
+        for (int c = 0; c < nchunks; ++c) start[c + 2] += start[c + 1];
+        for (int i = 0; i < count; ++i) {
+            const LevelDesc& L = levels[cands[i].level];
+            ord[start[L.chunk_off + key[i] / CMP_CHUNK + 1]++] = i;
+        }
+        for (int c = 0; c < nchunks; ++c)
+            std::sort(ord.begin() + (c > 0 ? start[c] : 0), ord.begin() + start[c + 1],
+                      [&](int a, int b) { return key[a] < key[b]; });
+        last_cands_.resize(count);
+        for (int i = 0; i < count; ++i) last_cands_[i] = cands[ord[i]];
+        if (checks) {
+            last_results_.resize(count);
+            for (int i = 0; i < count; ++i) last_results_[i] = res[ord[i]];
+        } else {
+            last_results_.clear();
+        }
+        cands = last_cands_.data();
+        res = checks ? last_results_.data() : nullptr;
+    }
     last_count_ = count;
     last_levels_ = levels;
-    last_cands_.assign(cands, cands + count);
-    if (checks) last_results_.assign(res, res + count);
-    else last_results_.clear();
 
     // Ordered replay of RunCheckDetect / CPUCheckDetect (cnn_detector_v3.cpp:1842-2011,
     // 1485-1818): the scan-time DropDetection against all detections so far, then per

@@ -20,13 +20,11 @@ hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8
 hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, const int2* l1_order, hipStream_t st,
+                         int* meta, Candidate* cands, const int2* l1_order, hipStream_t st,
                          const hipEvent_t* kev = nullptr);
 // L1-sum plane geometry and L1 tiles of a level whose response tiles are set
 // (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
 size_t level_l1_layout(LevelDesc& L);
-hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
-                          int* counts, int* meta, Candidate* cands, hipStream_t st);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
@@ -114,7 +112,6 @@ struct Slot {
     float* d_s1 = nullptr;       size_t s1_cap = 0;
     LevelDesc* d_levels = nullptr;
     int2* d_l1_order = nullptr;  size_t l1_order_cap = 0;
-    int* d_counts = nullptr;     size_t chunk_cap = 0;
     int* d_meta = nullptr;
     Candidate* d_cands = nullptr;
     CheckResult* d_results = nullptr;

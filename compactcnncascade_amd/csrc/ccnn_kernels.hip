@@ -11,11 +11,8 @@
 //   k_stage1_l1    pyramid sampling + L1 + neighbour sums   (cnn_simd_cntk.cpp:640-725,
 //                  of every level, once                       cnnpp_cplusplus.cpp)
 //   k_stage1       per 16x16 response tile: L2, L3, hidden   (cnn_simd_cntk.cpp:726-963)
-//                  and output layers; counts the candidate
-//                  windows (score > t1) per compaction chunk
-//   k_scatter
-//                  order-preserving compaction of the        (cnn_detector_v3.cpp:1918-1947)
-//                  candidate windows (levels in check order, raster order inside a level)
+//                  and output layers; appends the candidate  (cnn_detector_v3.cpp:1918-1947)
+//                  windows (score > t1) to the frame's list
 //   k_check        per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
 //                  resize to 49x51, u8, equalize, stage 2,    image_proc.cpp:357-368,720-800)
 //                  stage 3 on the mirrored crop when needed
@@ -219,25 +216,6 @@ constexpr int AX = S1A_TX, AY = S1A_TY;               // pooled L1 positions per
 constexpr int AW = 2 * AX + 3, AH = 2 * AY + 3;       // level window of one L1 tile
 }  // namespace s1
 
-__device__ __forceinline__ int block_excl_scan_256(int v, int* s_tmp, int* total)
-{
-    // exclusive scan of one int per thread over a 256-thread block
-    const int tid = threadIdx.x;
-    s_tmp[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int a = tid >= off ? s_tmp[tid - off] : 0;
-        __syncthreads();
-        s_tmp[tid] += a;
-        __syncthreads();
-    }
-    const int incl = s_tmp[tid];
-    *total = s_tmp[255];
-    __syncthreads();
-    return incl - v;
-}
-
-
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): workgroup b
 // runs on XCD b % 8. Renumber them so that each XCD takes one contiguous run of tiles,
 // whose input windows overlap and share that XCD's L2. A bijection on [0, n).
@@ -267,8 +245,7 @@ __device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n
 __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restrict__ g, int gp, int gw, int gh,
                                                            const LevelDesc* __restrict__ levels, int nlevels,
                                                            const float* __restrict__ W, float* __restrict__ s1buf,
-                                                           int* __restrict__ counts, int nchunks,
-                                                           const int2* __restrict__ order)
+                                                           int* __restrict__ meta, const int2* __restrict__ order)
 {
     using namespace s1;
     __shared__ float s_in[AH * AW];
@@ -276,8 +253,8 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     __shared__ float s_fx[AW], s_cx[AW], s_fy[AH], s_cy[AH];
     static_assert(AW + AH <= S1A_THREADS, "one table entry per thread");
     const int tid = threadIdx.x;
-    // clear the per-chunk candidate counts that k_stage1 accumulates
-    for (int c = blockIdx.x * S1A_THREADS + tid; c < nchunks; c += gridDim.x * S1A_THREADS) counts[c] = 0;
+    // the candidate count that k_stage1 accumulates starts at zero
+    if (meta && blockIdx.x == 0 && tid == 0) meta[0] = 0;
     // order (optional): the tiles as {level, tile within the level}, arranged so that the
     // contiguous run each XCD takes covers one horizontal band of every level, whose
     // gray-image rows then stay in that XCD's L2 across the levels
@@ -431,7 +408,7 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
 __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
                                                        const float* __restrict__ W,
                                                        const float* __restrict__ s1buf, float* __restrict__ resp,
-                                                       float t1, int* __restrict__ counts)
+                                                       float t1, int* __restrict__ meta, Candidate* __restrict__ cands)
 {
     using namespace s1;
     static_assert(TX * TY <= S1_THREADS, "one response value per thread");
@@ -600,58 +577,15 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const float out = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     const int e = (oy0 + oy) * L.rw + ox0 + ox;
     resp[L.resp_off + e] = out;
-    // candidate windows per compaction chunk (raster order within the level), for the
-    // order-preserving compaction
-    if (out > t1) atomicAdd(&counts[L.chunk_off + e / CMP_CHUNK], 1);
-}
-
-// ---------------------------------------------------------------------------------
-// Candidate compaction: responses > t1, levels in check order (the level table is in
-// that order), raster order inside a level.
-// ---------------------------------------------------------------------------------
-// One workgroup per chunk of CMP_CHUNK responses: the chunk's first output slot is the
-// sum of the counts of the chunks before it (read whole by every workgroup: a few
-// hundred to a few thousand ints); the last chunk's workgroup writes the total to meta[0].
-__global__ void __launch_bounds__(256) k_scatter(const float* __restrict__ resp, const LevelDesc* __restrict__ levels,
-                                                 int nlevels, float t1, const int* __restrict__ counts,
-                                                 Candidate* __restrict__ cands, int* __restrict__ meta)
-{
-    __shared__ int s_tmp[256];
-    const int c = blockIdx.x;
-    int before = 0;
-    for (int i = threadIdx.x; i < c; i += 256) before += counts[i];
-    {
-        int tot;
-        block_excl_scan_256(before, s_tmp, &tot);
-        before = tot;
-    }
-    const int li = find_level<BY_CHUNK>(levels, nlevels, c);
-    const LevelDesc L = levels[li];
-    const int n = L.rw * L.rh;
-    const int base = (c - L.chunk_off) * CMP_CHUNK;
-    constexpr int PER = CMP_CHUNK / 256;
-    float v[PER];
-    int cnt = 0;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int e = base + threadIdx.x * PER + k;
-        v[k] = e < n ? resp[L.resp_off + e] : 0.f;
-        if (e < n && v[k] > t1) cnt++;
-    }
-    int total;
-    int pos = before + block_excl_scan_256(cnt, s_tmp, &total);
-    if (c == gridDim.x - 1 && threadIdx.x == 0) meta[0] = before + total;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int e = base + threadIdx.x * PER + k;
-        if (e < n && v[k] > t1) {
-            Candidate cd;
-            cd.level = li;
-            cd.y = e / L.rw;
-            cd.x = e - cd.y * L.rw;
-            cd.score = v[k];
-            cands[pos++] = cd;
-        }
+    // candidate windows (score > t1) go to the frame's list in no particular order; the
+    // host replays them in the reference's order (level, then raster)
+    if (out > t1) {
+        Candidate cd;
+        cd.level = li;
+        cd.x = ox0 + ox;
+        cd.y = oy0 + oy;
+        cd.score = out;
+        cands[atomicAdd(&meta[0], 1)] = cd;
     }
 }
 
@@ -1398,31 +1332,21 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* counts, int nchunks, const int2* l1_order, hipStream_t st, const hipEvent_t* kev)
+                         int* meta, Candidate* cands, const int2* l1_order, hipStream_t st, const hipEvent_t* kev)
 {
     if (ntiles <= 0) return hipSuccess;
     // kev (optional): start/stop events of the two launches, stamped when each kernel
     // starts and ends executing
     if (kev)
         hipExtLaunchKernelGGL(k_stage1_l1, dim3(nl1tiles), dim3(S1A_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw,
-                              gh, d_levels, nlevels, W1, s1buf, counts, nchunks, l1_order);
+                              gh, d_levels, nlevels, W1, s1buf, meta, l1_order);
     else
-        k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, counts, nchunks,
-                                                      l1_order);
+        k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, meta, l1_order);
     if (kev)
         hipExtLaunchKernelGGL(k_stage1, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels, nlevels, W1,
-                              s1buf, resp, t1, counts);
+                              s1buf, resp, t1, meta, cands);
     else
-        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, counts);
-    return hipGetLastError();
-}
-
-hipError_t launch_compact(const float* resp, const LevelDesc* d_levels, int nlevels, int nchunks, float t1,
-                          int* counts, int* meta, Candidate* cands, hipStream_t st)
-{
-    // counts: per-chunk candidate counts accumulated by k_stage1
-    if (nchunks <= 0) return hipMemsetAsync(meta, 0, sizeof(int), st);
-    k_scatter<<<nchunks, 256, 0, st>>>(resp, d_levels, nlevels, t1, counts, cands, meta);
+        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands);
     return hipGetLastError();
 }
 
