@@ -407,7 +407,7 @@ void Detector::release_slot(Slot& s)
     dfree(s.d_l1_order);
 
     dfree(s.d_meta); dfree(s.d_cands); dfree(s.d_results); dfree(s.d_s2); dfree(s.d_s3);
-    hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results);
+    hfree(s.h_meta); hfree(s.h_cands); hfree(s.h_results); hfree(s.h_s23);
     for (auto& e : s.ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (auto& e : s.kev)
@@ -627,7 +627,9 @@ int Detector::ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& p
     }
     const size_t hc = std::min(std::max<size_t>(plan.resp_total, PREFETCH), PINNED_CAP);
     if (hc > s.h_cap) {
-        if (halloc(s.h_cands, s.hd_cands, hc) < 0 || halloc(s.h_results, s.hd_results, hc) < 0) return -1;
+        if (halloc(s.h_cands, s.hd_cands, hc) < 0 || halloc(s.h_results, s.hd_results, hc) < 0 ||
+            halloc(s.h_s23, s.hd_s23, 4 * hc) < 0)
+            return -1;
         s.h_cap = hc;
     }
     return 0;
@@ -665,6 +667,36 @@ int Detector::prepare(const ccnn_image& img)
     }
     if (!img.data) return -1;
     return 1;
+}
+
+// Decision of CPUCheckDetect (cnn_detector_v3.cpp:1709-1725, 1808-1811) from the stage-2/3
+// outcomes of both crop variants (s2/s3[mod] = {count above the threshold, max bits};
+// s3 {-1, ..} when stage 3 was skipped): mod 0, then mod 1 if mod 0 is rejected. The same
+// rule as k_check_decide.
+CheckResult check_decision(const CheckParams& P, float scale, const int2* s2, const int2* s3)
+{
+    const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
+    CheckResult res{};
+    res.n2 = 0;
+    res.n3 = 0;
+    for (int mod = 0; mod < 2; ++mod) {
+        const int2 a = s2[mod], b = s3[mod];
+        const int k1 = a.x, k2 = b.x < 0 ? 0 : b.x;
+        float m1, m2;
+        std::memcpy(&m1, &a.y, sizeof(float));
+        std::memcpy(&m2, &b.y, sizeof(float));
+        res.n2 += 1;
+        res.n3 += b.x < 0 ? 0 : 1;
+        bool flag = false;
+        if (P.type_check == 0) flag = (k1 >= mn || k2 >= mn);
+        else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
+        else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
+        res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
+        if (flag || !P.double_check) break;
+    }
+    res.fa.ok = 1;
+    res.fa.gender = res.fa.smile = res.fa.glasses = -1;
+    return res;
 }
 
 CheckParams Detector::check_params() const
@@ -714,32 +746,36 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
     }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[1], st));
     // k_stage1_l1 zeroes the candidate count and k_stage1 appends the candidates
+    s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
+    // candidates and check outcomes go straight to the mapped host buffers (up to s.pre):
+    // without facial analysis k_stage1 writes the candidates and k_check the stage-2/3
+    // outcomes there, and the host takes the decisions; with facial analysis (whose kernel
+    // reads the decisions) k_check_decide decides on the GPU and k_export copies; without
+    // checks k_export copies the candidates
+    s.pre = std::min(s.h_cap, s.cand_cap);
+    s.host_decide = s.checks && !set_.facial;
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
-                         s.d_resp, set_.t1, s.d_meta, s.d_cands, s.d_l1_order, st, profiling_ ? s.kev : nullptr));
+                         s.d_resp, set_.t1, s.d_meta, s.d_cands, s.host_decide ? s.hd_cands : nullptr, (int)s.pre,
+                         s.d_l1_order, st, profiling_ ? s.kev : nullptr));
     if (plan->ntiles <= 0) {
         HIP_OK(hipMemsetAsync(s.d_meta, 0, sizeof(int), st));
         if (profiling_)
             for (int k = 0; k < 4; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
     }
     if (profiling_) HIP_OK(hipEventRecord(s.ev[2], st));
-    s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
-    // candidates and check results go straight to the mapped host buffers (up to s.pre):
-    // written by the decision kernel, or by k_export after facial analysis or without checks
-    s.pre = std::min(s.h_cap, s.cand_cap);
-    const bool export_in_decide = s.checks && !set_.facial;
     if (s.checks) {
         HIP_OK(launch_check(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), d_w_[1],
                             d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_,
-                            export_in_decide ? s.hd_meta : nullptr, export_in_decide ? s.hd_cands : nullptr,
-                            export_in_decide ? s.hd_results : nullptr, (int)s.pre, st,
-                            profiling_ ? s.kev + 4 : nullptr));
+                            s.host_decide ? s.hd_meta : nullptr, nullptr, nullptr,
+                            s.host_decide ? s.hd_s23 : nullptr, s.host_decide ? s.hd_s23 + 2 * s.h_cap : nullptr,
+                            (int)s.pre, st, profiling_ ? s.kev + 4 : nullptr));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
                                  fa_blocks_, st));
     } else if (profiling_) {
         for (int k = 4; k < 6; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
     }
-    if (nl > 0 && !export_in_decide)
+    if (nl > 0 && !s.host_decide)
         HIP_OK(launch_export(s.d_meta, s.d_cands, s.checks ? s.d_results : nullptr, s.hd_meta, s.hd_cands,
                              s.hd_results, (int)s.pre, st));
     if (profiling_) HIP_OK(hipEventRecord(s.ev[3], st));
@@ -766,18 +802,35 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
     std::vector<CheckResult> big_r;
     const Candidate* cands = s.h_cands;
     const CheckResult* res = s.h_results;
-    if ((size_t)count > s.pre) {
+    const bool more = (size_t)count > s.pre;
+    if (more) {
         // more candidates than the mapped buffers hold: copy the whole list
-        {
-            big_c.resize(count);
-            HIP_OK(hipMemcpy(big_c.data(), s.d_cands, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
-            cands = big_c.data();
-            if (checks) {
-                big_r.resize(count);
-                HIP_OK(hipMemcpy(big_r.data(), s.d_results, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
-                res = big_r.data();
-            }
+        big_c.resize(count);
+        HIP_OK(hipMemcpy(big_c.data(), s.d_cands, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
+        cands = big_c.data();
+        if (checks && !s.host_decide) {
+            big_r.resize(count);
+            HIP_OK(hipMemcpy(big_r.data(), s.d_results, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
+            res = big_r.data();
         }
+    }
+    if (s.host_decide) {
+        // CPUCheckDetect's decision from the stage-2/3 outcomes, as k_check_decide takes it
+        std::vector<int2> big_s;
+        const int2* s2 = s.h_s23;
+        const int2* s3 = s.h_s23 + 2 * s.h_cap;
+        if (more) {
+            big_s.resize(4 * (size_t)count);
+            HIP_OK(hipMemcpy(big_s.data(), s.d_s2, sizeof(int2) * 2 * count, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(big_s.data() + 2 * count, s.d_s3, sizeof(int2) * 2 * count, hipMemcpyDeviceToHost));
+            s2 = big_s.data();
+            s3 = big_s.data() + 2 * count;
+        }
+        big_r.resize(count);
+        const CheckParams P = check_params();
+        for (int ci = 0; ci < count; ++ci)
+            big_r[ci] = check_decision(P, levels[cands[ci].level].scale, s2 + 2 * (size_t)ci, s3 + 2 * (size_t)ci);
+        res = big_r.data();
     }
     if (profiling_) {
         // gray, k_stage1_l1, k_stage1, compaction, k_check, decisions (+ facial analysis,
@@ -1077,7 +1130,7 @@ int Detector::stage_check(const float* gray, int gw, int gh, int n, const float*
         hipMemcpy(d_c, cd.data(), sizeof(Candidate) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_m, &n, sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
         launch_check(d_g, gp, gw, gh, d_l, d_c, d_m, check_params(), d_w_[1], d_w_[2], d_s2, d_s3, d_r,
-                     std::min(2 * n, 2048), nullptr, nullptr, nullptr, 0, st) != hipSuccess ||
+                     std::min(2 * n, 2048), nullptr, nullptr, nullptr, nullptr, nullptr, 0, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess ||
         hipMemcpy(out.data(), d_r, sizeof(CheckResult) * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = -1;

@@ -20,15 +20,16 @@ hipError_t launch_resize_u8(const uint8_t* src, int sw, int sh, int sstep, uint8
 hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDesc& L, float* out, hipStream_t st);
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* meta, Candidate* cands, const int2* l1_order, hipStream_t st,
-                         const hipEvent_t* kev = nullptr);
+                         int* meta, Candidate* cands, Candidate* hcands, int hcap, const int2* l1_order,
+                         hipStream_t st, const hipEvent_t* kev = nullptr);
 // L1-sum plane geometry and L1 tiles of a level whose response tiles are set
 // (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
 size_t level_l1_layout(LevelDesc& L);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
-                        CheckResult* hresults, int hcap, hipStream_t st, const hipEvent_t* kev = nullptr);
+                        CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st,
+                        const hipEvent_t* kev = nullptr);
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
@@ -121,12 +122,13 @@ struct Slot {
     int* h_meta = nullptr;            int* hd_meta = nullptr;
     Candidate* h_cands = nullptr;     Candidate* hd_cands = nullptr;
     CheckResult* h_results = nullptr; CheckResult* hd_results = nullptr;
+    int2* h_s23 = nullptr;            int2* hd_s23 = nullptr;   // stage-2 then stage-3 outcomes, 2 x h_cap each
     size_t h_cap = 0;
     const Plan* levels_plan = nullptr;   // plan whose level table is in d_levels
     // frame in flight
     const Plan* plan = nullptr;
     int gw = 0, gh = 0;
-    bool rescale = false, checks = false, busy = false;
+    bool rescale = false, checks = false, host_decide = false, busy = false;
     size_t pre = 0;
 };
 

@@ -408,7 +408,8 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
 __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
                                                        const float* __restrict__ W,
                                                        const float* __restrict__ s1buf, float* __restrict__ resp,
-                                                       float t1, int* __restrict__ meta, Candidate* __restrict__ cands)
+                                                       float t1, int* __restrict__ meta, Candidate* __restrict__ cands,
+                                                       Candidate* __restrict__ hcands, int hcap)
 {
     using namespace s1;
     static_assert(TX * TY <= S1_THREADS, "one response value per thread");
@@ -585,7 +586,10 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         cd.x = ox0 + ox;
         cd.y = oy0 + oy;
         cd.score = out;
-        cands[atomicAdd(&meta[0], 1)] = cd;
+        const int idx = atomicAdd(&meta[0], 1);
+        cands[idx] = cd;
+        // hcands (optional): the same record in mapped host memory, for the first hcap
+        if (hcands && idx < hcap) hcands[idx] = cd;
     }
 }
 
@@ -927,13 +931,17 @@ __device__ void write_input(ChkSmem& s, uint32_t (&u)[chk::PER], bool remap, boo
 // then, unless the stage-2 outcome ends the check early (cnn_detector_v3.cpp:1704-1706),
 // the stage-3 network on UCharToFloat_inv (mirror, image_proc.cpp:478-490) or offset 0.
 // s2res / s3res[2*ci + mod] = {count above the threshold, max response bits};
-// s3res {-1, -10.f} when stage 3 is skipped.
+// s3res {-1, -10.f} when stage 3 is skipped. hmeta / hs2 / hs3 (optional): the candidate
+// count and, for the first hcap candidates, the same outcomes in mapped host memory, for
+// decisions taken on the host.
 __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restrict__ g, int gp, int gw, int gh,
                                                          const LevelDesc* __restrict__ levels,
                                                          const Candidate* __restrict__ cands,
                                                          const int* __restrict__ meta, CheckParams P,
                                                          const float* __restrict__ W2, const float* __restrict__ W3,
-                                                         int2* __restrict__ s2res, int2* __restrict__ s3res)
+                                                         int2* __restrict__ s2res, int2* __restrict__ s3res,
+                                                         int* __restrict__ hmeta, int2* __restrict__ hs2,
+                                                         int2* __restrict__ hs3, int hcap)
 {
     using namespace chk;
     __shared__ ChkSmem s;
@@ -943,6 +951,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
     // each other so that they are dispatched together
     const int nv = P.double_check ? 2 : 1;
     const int items = meta[0] * nv;
+    if (hmeta && blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = meta[0];
     for (int item = blockIdx.x; item < items; item += gridDim.x) {
         const int ci = nv == 2 ? item >> 1 : item;
         const int mod = nv == 2 ? item & 1 : 0;
@@ -977,6 +986,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
                 if (d[i] > P.t2) k1++;
             }
             s2res[slot] = make_int2(k1, __float_as_int(m1));
+            if (hs2 && ci < hcap) hs2[slot] = make_int2(k1, __float_as_int(m1));
             s_k1 = k1;
         }
         __syncthreads();
@@ -985,7 +995,10 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         const bool skip3 = (P.type_check == 0 && k1 >= mn) || (P.type_check > 0 && k1 <= 0) ||
                            (P.type_check == 2 && k1 < mn);
         if (skip3) {
-            if (tid == 0) s3res[slot] = make_int2(-1, __float_as_int(-10.f));
+            if (tid == 0) {
+                s3res[slot] = make_int2(-1, __float_as_int(-10.f));
+                if (hs3 && ci < hcap) hs3[slot] = make_int2(-1, __float_as_int(-10.f));
+            }
             __syncthreads();
             continue;
         }
@@ -1008,6 +1021,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
                 if (d[i] > P.t3) k2++;
             }
             s3res[slot] = make_int2(k2, __float_as_int(m2));
+            if (hs3 && ci < hcap) hs3[slot] = make_int2(k2, __float_as_int(m2));
         }
         __syncthreads();
     }
@@ -1332,7 +1346,8 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
-                         int* meta, Candidate* cands, const int2* l1_order, hipStream_t st, const hipEvent_t* kev)
+                         int* meta, Candidate* cands, Candidate* hcands, int hcap, const int2* l1_order,
+                         hipStream_t st, const hipEvent_t* kev)
 {
     if (ntiles <= 0) return hipSuccess;
     // kev (optional): start/stop events of the two launches, stamped when each kernel
@@ -1344,9 +1359,9 @@ hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelD
         k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, meta, l1_order);
     if (kev)
         hipExtLaunchKernelGGL(k_stage1, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels, nlevels, W1,
-                              s1buf, resp, t1, meta, cands);
+                              s1buf, resp, t1, meta, cands, hcands, hcap);
     else
-        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands);
+        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands, hcands, hcap);
     return hipGetLastError();
 }
 
@@ -1395,16 +1410,22 @@ hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStre
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
-                        CheckResult* hresults, int hcap, hipStream_t st, const hipEvent_t* kev)
+                        CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st, const hipEvent_t* kev)
 {
-    // kev (optional): start/stop events of k_check, stamped when it starts and ends executing
+    // kev (optional): start/stop events of k_check, stamped when it starts and ends executing.
+    // With hs2/hs3 the outcomes go to mapped host memory and the host takes the decisions
+    // (no k_check_decide); otherwise k_check_decide writes results (and, with hmeta, the
+    // read-back).
+    int* km = hs2 ? hmeta : nullptr;
     if (kev)
         hipExtLaunchKernelGGL(k_check, dim3(nblocks), dim3(CHECK_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw, gh,
-                              d_levels, cands, meta, P, W2, W3, s2res, s3res);
+                              d_levels, cands, meta, P, W2, W3, s2res, s3res, km, hs2, hs3, hcap);
     else
-        k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res);
-    k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
-                                                          hcands, hresults, hcap);
+        k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res, km,
+                                                   hs2, hs3, hcap);
+    if (!hs2)
+        k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
+                                                              hcands, hresults, hcap);
     return hipGetLastError();
 }
 

@@ -139,7 +139,7 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out)
         return -1;
     if (ccnn::launch_stage1(g.as<float>(), lw, lw, lh, l.as<ccnn::LevelDesc>(), 1, L.l1_tiles_x * L.l1_tiles_y,
                             L.tiles_x * L.tiles_y, w.as<float>(), s1.as<float>(), r.as<float>(), INFINITY, nullptr,
-                            nullptr, nullptr, nullptr) != hipSuccess)
+                            nullptr, nullptr, 0, nullptr, nullptr) != hipSuccess)
         return -1;
     if (hipMemcpy(out, r.p, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return 0;
