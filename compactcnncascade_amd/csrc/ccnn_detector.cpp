@@ -709,7 +709,8 @@ CheckParams Detector::check_params() const
 }
 
 // Launches one frame on slot s: upload (host frames), optional 8-bit pre-resize, gray,
-// stage 1 over all levels, compaction, stage-2/3 checks, asynchronous read-back.
+// stage 1 over all levels (which appends the candidate windows), stage-2/3 checks,
+// asynchronous read-back.
 int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
 {
     HIP_OK(hipSetDevice(device_));
@@ -833,8 +834,8 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         res = big_r.data();
     }
     if (profiling_) {
-        // gray, k_stage1_l1, k_stage1, compaction, k_check, decisions (+ facial analysis,
-        // read-back), the whole frame
+        // gray, k_stage1_l1, k_stage1, the gap to k_check, k_check, decisions (+ facial
+        // analysis, read-back), the whole frame
         const hipEvent_t span[NT][2] = {{s.ev[0], s.ev[1]},  {s.kev[0], s.kev[1]}, {s.kev[2], s.kev[3]},
                                         {s.kev[3], s.ev[2]}, {s.kev[4], s.kev[5]}, {s.kev[5], s.ev[3]},
                                         {s.ev[0], s.ev[3]}};
@@ -844,7 +845,7 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
     }
     // k_stage1 appends the candidate windows in no particular order; the reference visits
     // them level by level (check order) and in raster order inside a level. Bucket them
-    // by compaction chunk (CMP_CHUNK consecutive raster positions of one level, chunks in
+    // by chunk (CMP_CHUNK consecutive raster positions of one level, chunks in
     // check order), then order each bucket by raster position.
     {
         const int nchunks = s.plan->nchunks;

@@ -100,8 +100,8 @@ struct Plan {
 // GPU work on frame i+1 while the host replays the decisions of frame i.
 struct Slot {
     hipStream_t st = nullptr;
-    // profiling events: on the stream at the frame's start, after gray, after the
-    // compaction and at the end (ev); start/stop of k_stage1_l1, k_stage1 and k_check,
+    // profiling events: on the stream at the frame's start, after gray, after stage 1
+    // and at the end (ev); start/stop of k_stage1_l1, k_stage1 and k_check,
     // stamped by the launches themselves (kev)
     static constexpr int NEV = 4, NKEV = 6;
     hipEvent_t ev[NEV] = {nullptr, nullptr, nullptr, nullptr};

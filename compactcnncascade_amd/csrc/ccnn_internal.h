@@ -56,8 +56,8 @@ struct LevelDesc {
     int resp_off;         // offset of the response map in the frame's response buffer
     int tile_off;         // first stage-1 tile of the level
     int tiles_x, tiles_y; // stage-1 tiles
-    int chunk_off;        // first compaction chunk of the level
-    int chunks;           // compaction chunks
+    int chunk_off;        // first candidate-ordering chunk of the level
+    int chunks;           // candidate-ordering chunks
     float scale;          // pyramid scale
     int s1_off;           // offset of the level's L1-sum planes in the frame's L1 buffer
     int s1_w, s1_h;       // L1-sum plane size: every pooled L1 position the tiles read
@@ -72,7 +72,8 @@ constexpr int S1A_TX = 64, S1A_TY = 16;
 constexpr int S1A_THREADS = 256;
 constexpr int S1_TX = 16, S1_TY = 16;
 constexpr int S1_THREADS = 256;
-// Compaction chunk (raster elements of one level per workgroup).
+// Chunk of raster positions of one level by which the host buckets the candidate list
+// before ordering it.
 constexpr int CMP_CHUNK = 1024;
 constexpr int CHECK_THREADS = 256;
 constexpr int FA_THREADS = 512;

@@ -227,16 +227,16 @@ __device__ __forceinline__ int xcd_tile(int b, int n)
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-// Level of global work item t: the last level whose first item (tile_off, chunk_off or
+// Level of global work item t: the last level whose first item (tile_off or
 // l1_tile_off) is <= t.
-enum { BY_TILE = 0, BY_CHUNK = 1, BY_L1_TILE = 2 };
+enum { BY_TILE = 0, BY_L1_TILE = 1 };
 template <int BY>
 __device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n, int t)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int off = BY == BY_TILE ? L[mid].tile_off : BY == BY_CHUNK ? L[mid].chunk_off : L[mid].l1_tile_off;
+        const int off = BY == BY_TILE ? L[mid].tile_off : L[mid].l1_tile_off;
         if (off <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
