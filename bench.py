@@ -40,9 +40,10 @@ def make_frames(name, n):
         return np.stack(list(ex.map(_frame, [(name, i) for i in range(n)])))
 
 
-def secondary_fps(api, torch, names, per_name, steps):
+def secondary_fps(api, torch, names, per_name, steps, facial=False):
     """Frames/s of DetectBatch over frames of the named workloads (all frames resident in
-    GPU memory, one detector initialised for the largest frame), best of `steps` passes."""
+    GPU memory, one detector initialised for the largest frame), best of `steps` passes;
+    facial: with facial analysis."""
     frames = []
     for name in names:
         frames.extend(make_frames(name, per_name))
@@ -53,7 +54,7 @@ def secondary_fps(api, torch, names, per_name, steps):
     H = max(f.shape[0] for f in frames)
     det = api.FaceDetector()
     if det.Init(api.Param(max_image_size=(W, H), min_face_height=cfg["min_face"], max_face_height=cfg["max_face"],
-                          scale_factor=cfg["scale_factor"])) != 0:
+                          scale_factor=cfg["scale_factor"], facial_analysis=facial)) != 0:
         raise SystemExit("FaceDetector.Init failed")
     df = [(t.data_ptr(), f.shape[1], f.shape[0], 3, f.shape[1] * 3) for t, f in zip(dev, frames)]
     det.DetectBatch(device_frames=df)
@@ -303,6 +304,8 @@ def main():
         # (4800x2400, scale 1.05, faces >= 15: 91 levels)
         line["vga_640x480"] = secondary_fps(api, torch, ["vga_min40"], 64, 3)
         line["deep_4800x2400"] = secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2)
+        # the headline workload with facial analysis (landmarks network on accepted windows)
+        line["fhd_facial"] = secondary_fps(api, torch, ["fhd_min40_max240"], 64, 3, facial=True)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(frames, dict(min_face=cfg["min_face"], max_face=cfg["max_face"],
                                        scale_factor=cfg["scale_factor"]), args.cpu_seconds)
