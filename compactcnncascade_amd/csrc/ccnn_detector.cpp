@@ -749,12 +749,12 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
     // k_stage1_l1 zeroes the candidate count and k_stage1 appends the candidates
     s.checks = set_.detect_mode != CCNN_DETECT_DISABLE && nl > 0;
     // candidates and check outcomes go straight to the mapped host buffers (up to s.pre):
-    // without facial analysis k_stage1 writes the candidates and k_check the stage-2/3
-    // outcomes there, and the host takes the decisions; with facial analysis (whose kernel
-    // reads the decisions) k_check_decide decides on the GPU and k_export copies; without
-    // checks k_export copies the candidates
+    // k_stage1 writes the candidates and k_check the stage-2/3 outcomes there, and the host
+    // takes the decisions; k_facial takes the same decisions itself and writes the facial
+    // results of the accepted candidates there; without checks k_export copies the
+    // candidates
     s.pre = std::min(s.h_cap, s.cand_cap);
-    s.host_decide = s.checks && !set_.facial;
+    s.host_decide = s.checks;
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
                          s.d_resp, set_.t1, s.d_meta, s.d_cands, s.host_decide ? s.hd_cands : nullptr, (int)s.pre,
                          s.d_l1_order, st, profiling_ ? s.kev : nullptr));
@@ -771,8 +771,8 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                             s.host_decide ? s.hd_s23 : nullptr, s.host_decide ? s.hd_s23 + 2 * s.h_cap : nullptr,
                             (int)s.pre, st, profiling_ ? s.kev + 4 : nullptr));
         if (set_.facial)
-            HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, d_w_[3], s.d_results,
-                                 fa_blocks_, st));
+            HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), s.d_s2,
+                                 s.d_s3, d_w_[3], s.d_results, s.hd_results, (int)s.pre, fa_blocks_, st));
     } else if (profiling_) {
         for (int k = 4; k < 6; ++k) HIP_OK(hipEventRecord(s.kev[k], st));
     }
@@ -809,11 +809,6 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         big_c.resize(count);
         HIP_OK(hipMemcpy(big_c.data(), s.d_cands, sizeof(Candidate) * count, hipMemcpyDeviceToHost));
         cands = big_c.data();
-        if (checks && !s.host_decide) {
-            big_r.resize(count);
-            HIP_OK(hipMemcpy(big_r.data(), s.d_results, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
-            res = big_r.data();
-        }
     }
     if (s.host_decide) {
         // CPUCheckDetect's decision from the stage-2/3 outcomes, as k_check_decide takes it
@@ -831,6 +826,18 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         const CheckParams P = check_params();
         for (int ci = 0; ci < count; ++ci)
             big_r[ci] = check_decision(P, levels[cands[ci].level].scale, s2 + 2 * (size_t)ci, s3 + 2 * (size_t)ci);
+        if (set_.facial) {
+            // k_facial's results of the accepted candidates
+            std::vector<CheckResult> dev_r;
+            const CheckResult* fr = s.h_results;
+            if (more) {
+                dev_r.resize(count);
+                HIP_OK(hipMemcpy(dev_r.data(), s.d_results, sizeof(CheckResult) * count, hipMemcpyDeviceToHost));
+                fr = dev_r.data();
+            }
+            for (int ci = 0; ci < count; ++ci)
+                if (big_r[ci].flag) big_r[ci].fa = fr[ci].fa;
+        }
         res = big_r.data();
     }
     if (profiling_) {

@@ -30,8 +30,11 @@ hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDe
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
                         CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st,
                         const hipEvent_t* kev = nullptr);
+// k_facial takes the check decisions from the stage-2/3 outcomes itself and writes the facial
+// results of the accepted candidates to results[].fa (and hresults[].fa below hcap)
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
-                         const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st);
+                         const int* meta, const CheckParams& P, const int2* s2res, const int2* s3res, const float* W,
+                         CheckResult* results, CheckResult* hresults, int hcap, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
 hipError_t launch_selftest_tanh(uint32_t lo, uint32_t hi, unsigned long long* out, hipStream_t st);
 hipError_t launch_export(const int* meta, const Candidate* cands, const CheckResult* results, int* hmeta,
@@ -118,7 +121,7 @@ struct Slot {
     CheckResult* d_results = nullptr;
     int2* d_s2 = nullptr;        int2* d_s3 = nullptr;
     size_t cand_cap = 0;
-    // mapped pinned host memory that k_export writes (h_*: host view, hd_*: device view)
+    // mapped pinned host memory that the kernels write (h_*: host view, hd_*: device view)
     int* h_meta = nullptr;            int* hd_meta = nullptr;
     Candidate* h_cands = nullptr;     Candidate* hd_cands = nullptr;
     CheckResult* h_results = nullptr; CheckResult* hd_results = nullptr;

@@ -16,9 +16,11 @@
 //   k_check        per (candidate, crop variant): crop, NN   (cnn_detector_v3.cpp:1485-1818,
 //                  resize to 49x51, u8, equalize, stage 2,    image_proc.cpp:357-368,720-800)
 //                  stage 3 on the mirrored crop when needed
-//   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1
+//   k_check_decide CPUCheckDetect's decision over mod 0 / mod 1 (stage-level entry; the
+//                  frame path decides on the host, k_facial on its own)
 //   k_resize_u8    8-bit bilinear pre-resize of 1-channel frames (image_resize.cpp:197-326)
-//   k_facial       landmarks network on the accepted windows     (cnn_detector_v3.cpp:1731-1754)
+//   k_facial       check decision + landmarks network on the      (cnn_detector_v3.cpp:1731-1754)
+//                  accepted windows
 //   k_landmarks    the landmarks network on one input (tests)
 //   k_export       read-back of candidates and results into mapped host memory
 //   k_selftest_tanh  the exact reciprocal against IEEE division over a range of floats
@@ -1028,7 +1030,37 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
 }
 
 // Decision of CPUCheckDetect (cnn_detector_v3.cpp:1709-1725, 1808-1811) from the stage-2/3
-// outcomes of both crop variants: mod 0, then mod 1 if mod 0 is rejected.
+// outcomes of both crop variants (s2/s3[mod] = {count above the threshold, max bits};
+// s3 {-1, ..} when stage 3 was skipped): mod 0, then mod 1 if mod 0 is rejected. The host
+// applies the same rule (check_decision) to the outcomes k_check writes to mapped memory.
+__device__ CheckResult check_decision_dev(const CheckParams& P, float scale, const int2* __restrict__ s2,
+                                          const int2* __restrict__ s3)
+{
+    const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
+    CheckResult res;
+    res.n2 = 0;
+    res.n3 = 0;
+    for (int mod = 0; mod < 2; ++mod) {
+        const int2 a = s2[mod];
+        const int2 b = s3[mod];
+        const int k1 = a.x, k2 = b.x < 0 ? 0 : b.x;
+        const float m1 = __int_as_float(a.y), m2 = __int_as_float(b.y);
+        res.n2 += 1;
+        res.n3 += b.x < 0 ? 0 : 1;
+        bool flag = false;
+        if (P.type_check == 0) flag = (k1 >= mn || k2 >= mn);
+        else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
+        else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
+        res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
+        if (flag || !P.double_check) break;
+    }
+    res.fa.ok = 1;
+    res.fa.gender = res.fa.smile = res.fa.glasses = -1;
+    return res;
+}
+
+// The decisions of all candidates, to results (and, with hmeta, the count, candidates and
+// results in mapped host memory for the first hcap).
 __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restrict__ levels,
                                                       const Candidate* __restrict__ cands,
                                                       const int* __restrict__ meta, CheckParams P,
@@ -1037,31 +1069,11 @@ __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restric
                                                       Candidate* __restrict__ hcands,
                                                       CheckResult* __restrict__ hresults, int hcap)
 {
-    // hmeta / hcands / hresults (optional): the read-back in mapped host memory, as k_export
     const int total = meta[0];
     if (hmeta && blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = total;
     for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < total; ci += gridDim.x * blockDim.x) {
-        const float scale = levels[cands[ci].level].scale;
-        const int mn = (P.adapt && scale > 0.7f && P.min_neighbors > 1) ? P.min_neighbors - 1 : P.min_neighbors;
-        CheckResult res;
-        res.n2 = 0;
-        res.n3 = 0;
-        for (int mod = 0; mod < 2; ++mod) {
-            const int2 a = s2res[(size_t)ci * 2 + mod];
-            const int2 b = s3res[(size_t)ci * 2 + mod];
-            const int k1 = a.x, k2 = b.x < 0 ? 0 : b.x;
-            const float m1 = __int_as_float(a.y), m2 = __int_as_float(b.y);
-            res.n2 += 1;
-            res.n3 += b.x < 0 ? 0 : 1;
-            bool flag = false;
-            if (P.type_check == 0) flag = (k1 >= mn || k2 >= mn);
-            else if (P.type_check == 1) flag = ((k1 >= mn && k2 > 0) || (k1 > 0 && k2 >= mn));
-            else if (P.type_check == 2) flag = (k1 >= mn && k2 >= mn);
-            res.flag = flag; res.mod = mod; res.k1 = k1; res.k2 = k2; res.m1 = m1; res.m2 = m2;
-            if (flag || !P.double_check) break;
-        }
-        res.fa.ok = 1;
-        res.fa.gender = res.fa.smile = res.fa.glasses = -1;
+        const CheckResult res = check_decision_dev(P, levels[cands[ci].level].scale, s2res + (size_t)ci * 2,
+                                                   s3res + (size_t)ci * 2);
         results[ci] = res;
         if (hresults && ci < hcap) {
             hresults[ci] = res;
@@ -1214,17 +1226,24 @@ __device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
 __global__ void __launch_bounds__(FA_THREADS) k_facial(const float* __restrict__ g, int gp, int gw, int gh,
                                                        const LevelDesc* __restrict__ levels,
                                                        const Candidate* __restrict__ cands,
-                                                       const int* __restrict__ meta, const float* __restrict__ W,
-                                                       CheckResult* __restrict__ results)
+                                                       const int* __restrict__ meta, CheckParams P,
+                                                       const int2* __restrict__ s2res, const int2* __restrict__ s3res,
+                                                       const float* __restrict__ W, CheckResult* __restrict__ results,
+                                                       CheckResult* __restrict__ hresults, int hcap)
 {
+    // each workgroup takes the check decision of its candidates (k_check's outcomes) and
+    // runs the landmarks network on the accepted ones; the facial result goes to
+    // results[ci].fa and, for ci < hcap, to the mapped hresults[ci].fa
     using namespace fa;
     __shared__ FaSmem s;
     const int tid = threadIdx.x;
     const int total = meta[0];
     for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
-        if (!results[ci].flag) continue;                  // uniform over the workgroup
         const Candidate cd = cands[ci];
-        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, results[ci].mod, gw, gh);
+        const float scale = levels[cd.level].scale;
+        const CheckResult res = check_decision_dev(P, scale, s2res + (size_t)ci * 2, s3res + (size_t)ci * 2);
+        if (!res.flag) continue;                          // uniform over the workgroup
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, res.mod, gw, gh);
         // nearest-neighbour resize of the crop to 49x51, a copy at that size
         // (image_resize.cpp:328-392, 564-584)
         const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
@@ -1263,6 +1282,7 @@ __global__ void __launch_bounds__(FA_THREADS) k_facial(const float* __restrict__
             f.smile = (int)(s.out[11] + 0.5f);
             f.glasses = (int)(s.out[12] + 0.5f);
             results[ci].fa = f;
+            if (hresults && ci < hcap) hresults[ci].fa = f;
         }
         __syncthreads();
     }
@@ -1389,9 +1409,11 @@ hipError_t launch_export(const int* meta, const Candidate* cands, const CheckRes
 }
 
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
-                         const int* meta, const float* W, CheckResult* results, int nblocks, hipStream_t st)
+                         const int* meta, const CheckParams& P, const int2* s2res, const int2* s3res, const float* W,
+                         CheckResult* results, CheckResult* hresults, int hcap, int nblocks, hipStream_t st)
 {
-    k_facial<<<nblocks, FA_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, W, results);
+    k_facial<<<nblocks, FA_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, s2res, s3res, W, results,
+                                             hresults, hcap);
     return hipGetLastError();
 }
 
