@@ -205,7 +205,7 @@ private:
     double acc_ms_[NT] = {};
     int acc_frames_ = 0;
     int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
-    int fa_blocks_ = 256;          // one 118 KB-LDS workgroup per CU
+    int fa_blocks_ = 64;           // k_facial workgroups (76 KB LDS each); windows dealt round-robin
 };
 
 }  // namespace ccnn

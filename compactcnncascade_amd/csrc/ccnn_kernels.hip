@@ -1085,7 +1085,7 @@ __global__ void __launch_bounds__(256) k_check_decide(const LevelDesc* __restric
 // ---------------------------------------------------------------------------------
 // Facial analysis (cnn_detector_v3.cpp:1731-1756): the landmarks network on the 49x51
 // FP32 crop (the nearest-neighbour resize before the u8 conversion) of every window the
-// checks accept. One workgroup of 512 threads per accepted candidate.
+// checks accept. One workgroup of 512 threads (two per CU) per accepted candidate.
 // ---------------------------------------------------------------------------------
 namespace fa {
 using N = Landmarks;
@@ -1098,23 +1098,67 @@ constexpr int G1 = 6;                                   // L1 maps per work item
 static_assert(P2W == N::K3W && P2H == N::K3H, "one L3 value per map");
 }  // namespace fa
 
+// LDS of one window (76 KB, small enough to share a CU with other kernels' workgroups):
+//   big  pooled L1, then (in place) the L1 sums; after L2 the L3, hidden- and output-layer
+//        products
+//   c    the network input; after L1 the pooled L2, then (in place) the L2 sums
 struct FaSmem {
-    float in[fa::H * fa::W];
-    float a[Landmarks::M1 * fa::P1N];    // pooled L1, later pooled L2
-    float b[Landmarks::M1 * fa::P1N];    // L1 sums, later L2 sums
+    alignas(16) float big[Landmarks::M1 * fa::P1N];
+    alignas(16) float c[Landmarks::M2 * fa::P2N > fa::H * fa::W ? Landmarks::M2 * fa::P2N : fa::H * fa::W];
+    __device__ float* in() { return c; }
     float a3[Landmarks::M3];
     float h[Landmarks::HL];
     float out[16];
 };
 
+// Sums of neighbouring maps (add / add2: a[0]+a[1], (a[i-1]+a[i])+a[i+1], a[M-2]+a[M-1])
+// of M maps of NP positions, in place: one work item per position reads all M maps of
+// its position into registers, then writes the M sums there.
+template <int M, int NP>
+__device__ __forceinline__ void fa_map_sums(float* a, const int tid)
+{
+    for (int p = tid; p < NP; p += FA_THREADS) {
+        float v[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) v[k] = a[k * NP + p];
+        a[p] = v[0] + v[1];
+#pragma unroll
+        for (int i = 1; i < M - 1; ++i) a[i * NP + p] = v[i - 1] + v[i] + v[i + 1];
+        a[(M - 1) * NP + p] = v[M - 2] + v[M - 1];
+    }
+}
+
+// Products x[xi(f)] * W[wi(f)] for f < NF into dst[di(f)]: a thread issues its weight
+// loads in batches of up to 16 before the batch's first product.
+template <int NF, class WI, class XI, class DI>
+__device__ __forceinline__ void fa_products(const float* __restrict__ W, WI wi, const float* __restrict__ x, XI xi,
+                                            float* __restrict__ dst, DI di, const int tid)
+{
+    constexpr int R = (NF + FA_THREADS - 1) / FA_THREADS;
+    constexpr int NB = (R + 15) / 16, RB = (R + NB - 1) / NB;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        float w[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) w[r] = W[wi(min(tid + (b * RB + r) * FA_THREADS, NF - 1))];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int f = tid + (b * RB + r) * FA_THREADS;
+            if (b * RB + r < R && f < NF) dst[di(f)] = x[xi(f)] * w[r];
+        }
+    }
+}
+
 // The landmarks network on s.in; all NOUT outputs to s.out. Layer arithmetic as in
 // check_forward: conv_4x4 / conv_3x3 chains, lrelu_bn(_max), add / add2 sums; L3 is
 // conv_11x10 (cnnpp_cplusplus.cpp:352-380), a chain that starts from 0.f; the hidden
 // layer is mulCN_add_tanhW over all 96 maps; the outputs are mulC / mulC1_add / tanhW.
-__device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
+__device__ __noinline__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
 {
     using namespace fa;
     const int tid = threadIdx.x;
+    float* const A = s.big;
+    float* const C = s.c;
     for (int it = tid; it < (N::M1 / G1) * P1N; it += FA_THREADS) {
         const int g = it / P1N, p = it - g * P1N;
         const int py = p / P1W, px = p - py * P1W;
@@ -1122,7 +1166,7 @@ __device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
 #pragma unroll
         for (int r = 0; r < 5; ++r)
 #pragma unroll
-            for (int c = 0; c < 5; ++c) win[r][c] = s.in[(2 * py + r) * fa::W + 2 * px + c];
+            for (int c = 0; c < 5; ++c) win[r][c] = C[(2 * py + r) * fa::W + 2 * px + c];
 #pragma unroll
         for (int mm = 0; mm < G1; ++mm) {
             const int m = g * G1 + mm;
@@ -1136,23 +1180,16 @@ __device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
                 for (int j = 1; j < 16; ++j) acc = acc + win[dy + (j >> 2)][dx + (j & 3)] * k[j];
                 a[q] = lrelu_bn(acc, W + N::oB1, m, N::M1);
             }
-            s.a[m * P1N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+            A[m * P1N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
     }
     __syncthreads();
-    for (int it = tid; it < N::M1 * P1N; it += FA_THREADS) {
-        const int i = it / P1N, p = it - i * P1N;
-        float v;
-        if (i == 0) v = s.a[p] + s.a[P1N + p];
-        else if (i == N::M1 - 1) v = s.a[(N::M1 - 2) * P1N + p] + s.a[(N::M1 - 1) * P1N + p];
-        else v = s.a[(i - 1) * P1N + p] + s.a[i * P1N + p] + s.a[(i + 1) * P1N + p];
-        s.b[it] = v;
-    }
+    fa_map_sums<N::M1, P1N>(A, tid);
     __syncthreads();
     for (int it = tid; it < N::M1 * P2N; it += FA_THREADS) {
         const int i = it / P2N, p = it - i * P2N;
         const int qy = p / P2W, qx = p - qy * P2W;
-        const float* src = s.b + i * P1N + (2 * qy) * P1W + 2 * qx;
+        const float* src = A + i * P1N + (2 * qy) * P1W + 2 * qx;
         float win[4][4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -1171,48 +1208,87 @@ __device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
                 for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
                 a[q] = lrelu_bn(acc, W + N::oB2, m, N::M2);
             }
-            s.a[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+            C[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
     }
     __syncthreads();
-    for (int it = tid; it < N::M2 * P2N; it += FA_THREADS) {
-        const int i = it / P2N, p = it - i * P2N;
-        float v;
-        if (i == 0) v = s.a[p] + s.a[P2N + p];
-        else if (i == N::M2 - 1) v = s.a[(N::M2 - 2) * P2N + p] + s.a[(N::M2 - 1) * P2N + p];
-        else v = s.a[(i - 1) * P2N + p] + s.a[i * P2N + p] + s.a[(i + 1) * P2N + p];
-        s.b[it] = v;
-    }
+    fa_map_sums<N::M2, P2N>(C, tid);
+    __syncthreads();
+    // L3, the hidden and the output layer are long dot products: all threads first form
+    // their products (coalesced weight reads) in LDS, then one thread per output runs the
+    // reference's chain over them in order (odd LDS strides: no bank conflicts).
+    constexpr int K3N = N::K3W * N::K3H, K3S = K3N + 1;
+    static_assert(N::M3 * K3S <= N::M1 * P1N, "L3 products fit");
+    // f = m * K3N + j: (m >> 1) * P2N + j reads sum map m / 2 (P2N == K3N)
+    fa_products<N::M3 * K3N>(
+        W, [](int f) { return N::oK3 + f; }, C, [](int f) { return (f / (2 * K3N)) * K3N + f % K3N; }, A,
+        [](int f) { return f + f / K3N; }, tid);
     __syncthreads();
     if (tid < N::M3) {
-        const int m = tid, i = m >> 1;
-        const float* src = s.b + i * P2N;
-        const float* k = W + N::oK3 + m * N::K3W * N::K3H;
+        const float* p = A + tid * K3S;
         float d = 0.f;
-        for (int j = 0; j < N::K3W * N::K3H; ++j) d = d + src[j] * k[j];
-        s.a3[m] = lrelu_bn(d, W + N::oB3, m, N::M3);
+#pragma unroll 10
+        for (int j = 0; j < K3N; ++j) d = d + p[j];
+        s.a3[tid] = lrelu_bn(d, W + N::oB3, tid, N::M3);
     }
     __syncthreads();
-    for (int u = tid; u < N::HL; u += FA_THREADS) {
-        const int gi = u / N::HLS, tt = u - gi * N::HLS;
-        const float* w = W + N::oHW + (size_t)u * N::HLN;
-        float c = W[N::oHB + u];
-        for (int j = 0; j < N::HLN; ++j) c = c + s.a3[gi + j] * w[j];
-        c = c * W[N::oHT + gi];
-        c = rtanh(c);
-        c = (c + 1.f) * 0.5f;
-        s.h[u] = W[N::oHBW + tt] * c + W[N::oHBB + tt];
+    // hidden layer, its dot products in three parts
+    constexpr int UPT = (N::HL + FA_THREADS - 1) / FA_THREADS;   // units per thread
+    constexpr int JC = N::HLN / 3, JS = JC + 1;
+    static_assert(N::HLN % 3 == 0 && N::HL * JS <= N::M1 * P1N, "hidden products fit");
+    float c[UPT];
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) c[k] = W[N::oHB + min(tid + k * FA_THREADS, N::HL - 1)];
+#pragma unroll
+    for (int j0 = 0; j0 < N::HLN; j0 += JC) {
+        // f = u * JC + jj: unit u's weight j0 + jj
+        fa_products<N::HL * JC>(
+            W, [j0](int f) { return N::oHW + (f / JC) * N::HLN + j0 + f % JC; }, s.a3 + j0,
+            [](int f) { return (f / JC) / N::HLS + f % JC; }, A, [](int f) { return f + f / JC; }, tid);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < UPT; ++k) {
+            const int u = tid + k * FA_THREADS;
+            if (u < N::HL) {
+                const float* p = A + u * JS;
+#pragma unroll 8
+                for (int jj = 0; jj < JC; ++jj) c[k] = c[k] + p[jj];
+            }
+        }
+        __syncthreads();
     }
+#pragma unroll
+    for (int k = 0; k < UPT; ++k) {
+        const int u = tid + k * FA_THREADS;
+        if (u < N::HL) {
+            const int gi = u / N::HLS, tt = u - gi * N::HLS;
+            float v = c[k] * W[N::oHT + gi];
+            v = rtanh(v);
+            v = (v + 1.f) * 0.5f;
+            s.h[u] = W[N::oHBW + tt] * v + W[N::oHBB + tt];
+        }
+    }
+    __syncthreads();
+    // output layer: products in chain order q = HLS * gi + tt (gi outer, tt inner), rows
+    // 16-byte aligned for float4 reads by the chains
+    constexpr int OS = N::HL + 4;
+    static_assert(N::HL % 4 == 0 && N::NOUT * OS <= N::M1 * P1N, "output products fit");
+    fa_products<N::NOUT * N::HL>(
+        W, [](int f) { const int o = f / N::HL, q = f % N::HL;
+                       return N::oOW + o * N::HL + (q % N::HLS) * N::HLG + q / N::HLS; },
+        s.h, [](int f) { return f % N::HL; }, A, [](int f) { return f + 4 * (f / N::HL); }, tid);
     __syncthreads();
     if (tid < N::NOUT) {
-        const float* w = W + N::oOW + (size_t)tid * N::HL;
+        const float4* p4 = reinterpret_cast<const float4*>(A + tid * OS);
         float o = 0.f;
-        for (int gi = 0; gi < N::HLG; ++gi)
-            for (int tt = 0; tt < N::HLS; ++tt) {
-                const float wv = w[tt * N::HLG + gi];
-                if (gi == 0 && tt == 0) o = s.h[0] * wv;
-                else o = s.h[N::HLS * gi + tt] * wv + o;
-            }
+#pragma unroll 4
+        for (int k = 0; k < N::HL / 4; ++k) {
+            const float4 v = p4[k];
+            o = (k == 0) ? v.x : v.x + o;
+            o = v.y + o;
+            o = v.z + o;
+            o = v.w + o;
+        }
         float c = o + W[N::oOB + tid];
         c = c * W[N::oOB + N::NOUT];
         c = rtanh(c);
@@ -1223,77 +1299,114 @@ __device__ void landmarks_forward(const float* __restrict__ W, FaSmem& s)
 }
 
 // Facial analysis of every accepted candidate, on the crop of its final crop variant.
-__global__ void __launch_bounds__(FA_THREADS) k_facial(const float* __restrict__ g, int gp, int gw, int gh,
-                                                       const LevelDesc* __restrict__ levels,
-                                                       const Candidate* __restrict__ cands,
-                                                       const int* __restrict__ meta, CheckParams P,
-                                                       const int2* __restrict__ s2res, const int2* __restrict__ s3res,
-                                                       const float* __restrict__ W, CheckResult* __restrict__ results,
-                                                       CheckResult* __restrict__ hresults, int hcap)
+// Every workgroup takes the check decisions of all candidates (k_check's outcomes), in
+// chunks of FA_THREADS candidates, and numbers the accepted ones in candidate order;
+// workgroup b runs the landmarks network on the accepted windows whose number is b modulo
+// the grid size. The facial result goes to results[ci].fa and, for ci < hcap, to the
+// mapped hresults[ci].fa.
+__global__ void __launch_bounds__(FA_THREADS, 4) k_facial(const float* __restrict__ g, int gp, int gw, int gh,
+                                                          const LevelDesc* __restrict__ levels,
+                                                          const Candidate* __restrict__ cands,
+                                                          const int* __restrict__ meta, CheckParams P,
+                                                          const int2* __restrict__ s2res,
+                                                          const int2* __restrict__ s3res,
+                                                          const float* __restrict__ W,
+                                                          CheckResult* __restrict__ results,
+                                                          CheckResult* __restrict__ hresults, int hcap)
 {
-    // each workgroup takes the check decision of its candidates (k_check's outcomes) and
-    // runs the landmarks network on the accepted ones; the facial result goes to
-    // results[ci].fa and, for ci < hcap, to the mapped hresults[ci].fa
     using namespace fa;
     __shared__ FaSmem s;
-    const int tid = threadIdx.x;
+    __shared__ int s_list[FA_THREADS];                    // accepted candidates of the chunk
+    __shared__ int s_mod[FA_THREADS];                     // and their final crop variants
+    __shared__ int s_wcnt[FA_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int total = meta[0];
-    for (int ci = blockIdx.x; ci < total; ci += gridDim.x) {
-        const Candidate cd = cands[ci];
-        const float scale = levels[cd.level].scale;
-        const CheckResult res = check_decision_dev(P, scale, s2res + (size_t)ci * 2, s3res + (size_t)ci * 2);
-        if (!res.flag) continue;                          // uniform over the workgroup
-        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, res.mod, gw, gh);
-        // nearest-neighbour resize of the crop to 49x51, a copy at that size
-        // (image_resize.cpp:328-392, 564-584)
-        const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
-        const uint32_t sclx = ((uint32_t)cr.rcols << 12) / fa::W + 1;
-        const uint32_t scly = ((uint32_t)cr.rrows << 12) / fa::H + 1;
-        constexpr int PER = (fa::W * fa::H + FA_THREADS - 1) / FA_THREADS;
-        float v[PER];
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = min(tid + k * FA_THREADS, fa::W * fa::H - 1);
-            const int iy = i / fa::W, ix = i - iy * fa::W;
-            uint32_t px = ix, py = iy;
-            if (!copy) {
-                px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
-                py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
-            }
-            v[k] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
+    const int G = gridDim.x;
+    int base = 0;                                         // accepted before this chunk
+    for (int c0 = 0; c0 < total; c0 += FA_THREADS) {
+        const int ci = c0 + tid;
+        bool acc = false;
+        int mod = 0;
+        if (ci < total) {
+            const CheckResult res = check_decision_dev(P, levels[cands[ci].level].scale, s2res + (size_t)ci * 2,
+                                                       s3res + (size_t)ci * 2);
+            acc = res.flag != 0;
+            mod = res.mod;
         }
+        const unsigned long long bal = __ballot(acc);
+        if (lane == 0) s_wcnt[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0, n = 0;
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * FA_THREADS;
-            if (i < fa::W * fa::H) s.in[i] = v[k];
+        for (int k = 0; k < FA_THREADS / 64; ++k) {
+            off += k < wave ? s_wcnt[k] : 0;
+            n += s_wcnt[k];
+        }
+        if (acc) {
+            const int k = off + __popcll(bal & ((1ull << lane) - 1));
+            s_list[k] = ci;
+            s_mod[k] = mod;
         }
         __syncthreads();
-        landmarks_forward(W, s);
-        if (tid == 0) {
-            int zero = 0;
-            for (int t = 0; t < 5; ++t) {
-                const int lx = (int)(s.out[2 * t] * (float)cr.rcols);
-                const int ly = (int)(s.out[2 * t + 1] * (float)cr.rrows);
-                if (lx < (cr.rcols >> 1) && ly < (cr.rrows >> 1)) zero++;
+        // this workgroup's windows of the chunk: numbers base + k with (base + k) % G == b
+        for (int k = (int)(((unsigned)blockIdx.x + (unsigned)G - (unsigned)(base % G)) % (unsigned)G); k < n;
+             k += G) {
+            const int wi = s_list[k];
+            const Candidate cd = cands[wi];
+            const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, s_mod[k], gw, gh);
+            // nearest-neighbour resize of the crop to 49x51, a copy at that size
+            // (image_resize.cpp:328-392, 564-584)
+            const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
+            const uint32_t sclx = ((uint32_t)cr.rcols << 12) / fa::W + 1;
+            const uint32_t scly = ((uint32_t)cr.rrows << 12) / fa::H + 1;
+            constexpr int PER = (fa::W * fa::H + FA_THREADS - 1) / FA_THREADS;
+            float v[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int i = min(tid + q * FA_THREADS, fa::W * fa::H - 1);
+                const int iy = i / fa::W, ix = i - iy * fa::W;
+                uint32_t px = ix, py = iy;
+                if (!copy) {
+                    px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
+                    py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
+                }
+                v[q] = g[(size_t)(cr.ry + py) * gp + cr.rx + px];
             }
-            FaResult f;
-            f.ok = zero != 5;
-            f.gender = (int)(s.out[10] + 0.5f);
-            f.smile = (int)(s.out[11] + 0.5f);
-            f.glasses = (int)(s.out[12] + 0.5f);
-            results[ci].fa = f;
-            if (hresults && ci < hcap) hresults[ci].fa = f;
+            float* in = s.in();
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int i = tid + q * FA_THREADS;
+                if (i < fa::W * fa::H) in[i] = v[q];
+            }
+            __syncthreads();
+            landmarks_forward(W, s);
+            if (tid == 0) {
+                int zero = 0;
+                for (int t = 0; t < 5; ++t) {
+                    const int lx = (int)(s.out[2 * t] * (float)cr.rcols);
+                    const int ly = (int)(s.out[2 * t + 1] * (float)cr.rrows);
+                    if (lx < (cr.rcols >> 1) && ly < (cr.rrows >> 1)) zero++;
+                }
+                FaResult f;
+                f.ok = zero != 5;
+                f.gender = (int)(s.out[10] + 0.5f);
+                f.smile = (int)(s.out[11] + 0.5f);
+                f.glasses = (int)(s.out[12] + 0.5f);
+                results[wi].fa = f;
+                if (hresults && wi < hcap) hresults[wi].fa = f;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        base += n;
     }
 }
 
-// The landmarks network on one 49x51 input (stage-level entry point).
-__global__ void __launch_bounds__(FA_THREADS) k_landmarks(const float* __restrict__ in,
-                                                          const float* __restrict__ W, float* __restrict__ out)
+// The landmarks network on one 49x51 input (tests).
+__global__ void __launch_bounds__(FA_THREADS, 4) k_landmarks(const float* __restrict__ in,
+                                                             const float* __restrict__ W, float* __restrict__ out)
 {
     __shared__ FaSmem s;
-    for (int i = threadIdx.x; i < fa::W * fa::H; i += FA_THREADS) s.in[i] = in[i];
+    for (int i = threadIdx.x; i < fa::W * fa::H; i += FA_THREADS) s.in()[i] = in[i];
     __syncthreads();
     landmarks_forward(W, s);
     if (threadIdx.x < Landmarks::NOUT) out[threadIdx.x] = s.out[threadIdx.x];
