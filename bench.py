@@ -160,6 +160,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the 4K and mixed-size batch measurements reported beside `value`")
+    ap.add_argument("--facial", action="store_true",
+                    help="run the workload with facial analysis (the line's config says so)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -186,7 +188,7 @@ def main():
 
     det = api.FaceDetector()
     param = api.Param(max_image_size=(W, H), min_face_height=cfg["min_face"], max_face_height=cfg["max_face"],
-                      scale_factor=cfg["scale_factor"])
+                      scale_factor=cfg["scale_factor"], facial_analysis=args.facial)
     if det.Init(param) != 0:
         raise SystemExit("FaceDetector.Init failed")
 
@@ -273,6 +275,7 @@ def main():
         "data": "synthetic (seeded generator, compactcnncascade_amd/synth.py)",
         "config": {"workload": args.workload, "frames_per_step": args.frames, "frame": "%dx%d BGR u8" % (W, H),
                    "scale_factor": cfg["scale_factor"], "min_face": cfg["min_face"], "max_face": cfg["max_face"],
+                   "facial_analysis": args.facial,
                    "launch": "per-frame launches, consecutive frames pipelined over three streams, "
                              "frames resident in HBM"},
         "roofline": {

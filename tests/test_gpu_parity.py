@@ -256,6 +256,31 @@ def test_facial_analysis_per_candidate_matches_oracle(api, oracle):
     assert ev.sum() > 5 and (tr["c_status"] == 4).sum() > 0
 
 
+def test_facial_analysis_on_a_crowded_frame_matches_oracle(api, oracle):
+    # more candidates than one facial workgroup decides at a time (512) and more accepted
+    # windows than facial workgroups (64): every window analysed once, verdicts and the
+    # faces' attributes as the oracle's
+    from compactcnncascade_amd import synth
+    f = synth.synth_frame(5, 1280, 720, 80.0, 24, 90)[0]
+    over = dict(min_face=20, facial_analysis=1)
+    exp, tr = oracle.detect(f, params_for(f, over), trace_cap=200000)
+    ev = np.isin(tr["c_status"], [2, 4])
+    assert len(ev) > 512 and ev.sum() > 64
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    n, faces = d.Detect(f)
+    t = d.last_trace()
+    fa = d.last_facial()
+    assert len(fa) == len(ev)
+    assert np.array_equal(t["res"][:, 0] == 1, ev)
+    assert np.array_equal(fa[ev, 0] == 1, tr["c_status"][ev] == 2)
+    assert n == len(exp)
+    got = [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"]), int(r["gender"]), int(r["smile"]),
+            int(r["glasses"])] for r in faces]
+    assert got == [[int(v) for v in e[:4]] + [int(v) for v in e[5:8]] for e in exp]
+    assert bits(faces["score"]).tolist() == bits(np.asarray(exp[:, 4], np.float32)).tolist()
+
+
 def test_repeated_detect_is_stable(api):
     from cases import vga
     f = vga(0)
