@@ -246,7 +246,7 @@ def main():
     e2 /= len(records)
     e3 /= len(records)
     per_frame_ms = {k: tm["totals"][k] / max(1, tm["frames"])
-                    for k in ("gray", "stage1_l1", "stage1", "compact", "check", "decide")}
+                    for k in ("gray", "stage1_l1", "stage1", "gap", "check", "after_check")}
     # algorithmic FLOP (2 per MAC) per launch of the three VALU kernels, and what their
     # event-timed launches achieve
     kflops = {"k_stage1_l1": 2.0 * s1[0], "k_stage1": 2.0 * s1[1],

@@ -301,10 +301,13 @@ class FaceDetector:
         self._L.ccnn_set_profiling(self._h, int(on))
 
     def last_timing(self):
-        """Per-kernel milliseconds of the last frame, and totals since set_profiling."""
+        """Per-kernel milliseconds of the last frame, and totals since set_profiling: gray,
+        stage1_l1 (k_stage1_l1), stage1 (k_stage1), gap (k_stage1's end to k_check's start),
+        check (k_check), after_check (facial analysis, or the read-back without checks),
+        total."""
         ms = np.zeros(15, np.float32)
         self._L.ccnn_last_timing(self._h, _ptr(ms))
-        keys = ("gray", "stage1_l1", "stage1", "compact", "check", "decide", "total")
+        keys = ("gray", "stage1_l1", "stage1", "gap", "check", "after_check", "total")
         last = {k: float(ms[i]) for i, k in enumerate(keys)}
         totals = {k: float(ms[7 + i]) for i, k in enumerate(keys)}
         return dict(last=last, totals=totals, frames=int(ms[14]))

@@ -183,7 +183,8 @@ int ccnn_last_error(int reset);
  * (and the totals reset) by ccnn_set_profiling(d, 1). ms[0..6] = gray (with upload and
  * pre-resize), stage-1 L1 sums (k_stage1_l1), stage-1 tiles (k_stage1, which also
  * appends the candidate list), the gap from k_stage1 to k_check, stage-2/3 networks
- * (k_check), check decisions and facial analysis, the whole frame,
+ * (k_check), what runs after k_check (facial analysis when on; the candidate read-back
+ * when the checks are off), the whole frame,
  * for the last frame; ms[7..13] the same summed over the frames since profiling was
  * enabled; ms[14] that frame count. ms must hold 15 floats. Returns 0. */
 void ccnn_set_profiling(ccnn_detector* d, int on);
