@@ -1128,24 +1128,30 @@ __device__ __forceinline__ void fa_map_sums(float* a, const int tid)
     }
 }
 
-// Products x[xi(f)] * W[wi(f)] for f < NF into dst[di(f)]: a thread issues its weight
-// loads in batches of up to 16 before the batch's first product.
-template <int NF, class WI, class XI, class DI>
-__device__ __forceinline__ void fa_products(const float* __restrict__ W, WI wi, const float* __restrict__ x, XI xi,
-                                            float* __restrict__ dst, DI di, const int tid)
+// Dot-product terms x[xi(f)] * W[wi(f)], f < NF, formed in parallel: a thread's weights
+// (f = tid + r * FA_THREADS) are loaded into registers by fa_load, one phase ahead of
+// their use, and multiplied into dst[di(f)] by fa_mul.
+template <int NF>
+struct FaWeights {
+    static constexpr int R = (NF + FA_THREADS - 1) / FA_THREADS;
+    float w[R];
+};
+
+template <int NF, class WI>
+__device__ __forceinline__ void fa_load(FaWeights<NF>& fw, const float* __restrict__ W, WI wi, const int tid)
 {
-    constexpr int R = (NF + FA_THREADS - 1) / FA_THREADS;
-    constexpr int NB = (R + 15) / 16, RB = (R + NB - 1) / NB;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        float w[RB];
+    for (int r = 0; r < FaWeights<NF>::R; ++r) fw.w[r] = W[wi(min(tid + r * FA_THREADS, NF - 1))];
+}
+
+template <int NF, class XI, class DI>
+__device__ __forceinline__ void fa_mul(const FaWeights<NF>& fw, const float* __restrict__ x, XI xi,
+                                       float* __restrict__ dst, DI di, const int tid)
+{
 #pragma unroll
-        for (int r = 0; r < RB; ++r) w[r] = W[wi(min(tid + (b * RB + r) * FA_THREADS, NF - 1))];
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const int f = tid + (b * RB + r) * FA_THREADS;
-            if (b * RB + r < R && f < NF) dst[di(f)] = x[xi(f)] * w[r];
-        }
+    for (int r = 0; r < FaWeights<NF>::R; ++r) {
+        const int f = tid + r * FA_THREADS;
+        if (f < NF) dst[di(f)] = x[xi(f)] * fw.w[r];
     }
 }
 
@@ -1212,18 +1218,37 @@ __device__ __noinline__ void landmarks_forward(const float* __restrict__ W, FaSm
         }
     }
     __syncthreads();
-    fa_map_sums<N::M2, P2N>(C, tid);
-    __syncthreads();
     // L3, the hidden and the output layer are long dot products: all threads first form
     // their products (coalesced weight reads) in LDS, then one thread per output runs the
-    // reference's chain over them in order (odd LDS strides: no bank conflicts).
+    // reference's chain over them in order (odd LDS strides: no bank conflicts). Each
+    // phase's weights are loaded during the phase before it.
     constexpr int K3N = N::K3W * N::K3H, K3S = K3N + 1;
     static_assert(N::M3 * K3S <= N::M1 * P1N, "L3 products fit");
-    // f = m * K3N + j: (m >> 1) * P2N + j reads sum map m / 2 (P2N == K3N)
-    fa_products<N::M3 * K3N>(
-        W, [](int f) { return N::oK3 + f; }, C, [](int f) { return (f / (2 * K3N)) * K3N + f % K3N; }, A,
-        [](int f) { return f + f / K3N; }, tid);
+    constexpr int UPT = (N::HL + FA_THREADS - 1) / FA_THREADS;   // hidden units per thread
+    constexpr int JC = N::HLN / 3, JS = JC + 1;                  // hidden dot products in 3 parts
+    static_assert(N::HLN % 3 == 0 && N::HL * JS <= N::M1 * P1N, "hidden products fit");
+    constexpr int OS = N::HL + 4;                                // output rows 16-byte aligned
+    static_assert(N::HL % 4 == 0 && N::NOUT * OS <= N::M1 * P1N, "output products fit");
+    // f = m * K3N + j: L3 map m's weight j, times sum map m / 2 at j (P2N == K3N)
+    const auto l3_w = [](int f) { return N::oK3 + f; };
+    const auto l3_x = [](int f) { return (f / (2 * K3N)) * K3N + f % K3N; };
+    const auto l3_d = [](int f) { return f + f / K3N; };
+    // f = u * JC + jj: hidden unit u's weight j0 + jj, times a3[u / HLS + j0 + jj]
+    const auto hl_x = [](int f) { return (f / JC) / N::HLS + f % JC; };
+    const auto hl_d = [](int f) { return f + f / JC; };
+    // f = o * HL + q: output o's term q of its chain (q = HLS * gi + tt), times h[q]
+    const auto ol_w = [](int f) {
+        const int o = f / N::HL, q = f % N::HL;
+        return N::oOW + o * N::HL + (q % N::HLS) * N::HLG + q / N::HLS;
+    };
+    FaWeights<N::M3 * K3N> w3;
+    fa_load(w3, W, l3_w, tid);
+    fa_map_sums<N::M2, P2N>(C, tid);
     __syncthreads();
+    fa_mul(w3, C, l3_x, A, l3_d, tid);
+    __syncthreads();
+    FaWeights<N::HL * JC> wh[2];
+    fa_load(wh[0], W, [](int f) { return N::oHW + (f / JC) * N::HLN + f % JC; }, tid);
     if (tid < N::M3) {
         const float* p = A + tid * K3S;
         float d = 0.f;
@@ -1232,20 +1257,21 @@ __device__ __noinline__ void landmarks_forward(const float* __restrict__ W, FaSm
         s.a3[tid] = lrelu_bn(d, W + N::oB3, tid, N::M3);
     }
     __syncthreads();
-    // hidden layer, its dot products in three parts
-    constexpr int UPT = (N::HL + FA_THREADS - 1) / FA_THREADS;   // units per thread
-    constexpr int JC = N::HLN / 3, JS = JC + 1;
-    static_assert(N::HLN % 3 == 0 && N::HL * JS <= N::M1 * P1N, "hidden products fit");
+    // hidden layer
     float c[UPT];
 #pragma unroll
     for (int k = 0; k < UPT; ++k) c[k] = W[N::oHB + min(tid + k * FA_THREADS, N::HL - 1)];
+    FaWeights<N::NOUT * N::HL> wo;
 #pragma unroll
-    for (int j0 = 0; j0 < N::HLN; j0 += JC) {
-        // f = u * JC + jj: unit u's weight j0 + jj
-        fa_products<N::HL * JC>(
-            W, [j0](int f) { return N::oHW + (f / JC) * N::HLN + j0 + f % JC; }, s.a3 + j0,
-            [](int f) { return (f / JC) / N::HLS + f % JC; }, A, [](int f) { return f + f / JC; }, tid);
+    for (int part = 0; part < 3; ++part) {
+        const int j0 = part * JC;
+        fa_mul(wh[part & 1], s.a3 + j0, hl_x, A, hl_d, tid);
         __syncthreads();
+        if (part < 2)
+            fa_load(wh[(part + 1) & 1], W,
+                    [j0](int f) { return N::oHW + (f / JC) * N::HLN + j0 + JC + f % JC; }, tid);
+        else
+            fa_load(wo, W, ol_w, tid);
 #pragma unroll
         for (int k = 0; k < UPT; ++k) {
             const int u = tid + k * FA_THREADS;
@@ -1269,14 +1295,8 @@ __device__ __noinline__ void landmarks_forward(const float* __restrict__ W, FaSm
         }
     }
     __syncthreads();
-    // output layer: products in chain order q = HLS * gi + tt (gi outer, tt inner), rows
-    // 16-byte aligned for float4 reads by the chains
-    constexpr int OS = N::HL + 4;
-    static_assert(N::HL % 4 == 0 && N::NOUT * OS <= N::M1 * P1N, "output products fit");
-    fa_products<N::NOUT * N::HL>(
-        W, [](int f) { const int o = f / N::HL, q = f % N::HL;
-                       return N::oOW + o * N::HL + (q % N::HLS) * N::HLG + q / N::HLS; },
-        s.h, [](int f) { return f % N::HL; }, A, [](int f) { return f + 4 * (f / N::HL); }, tid);
+    // output layer
+    fa_mul(wo, s.h, [](int f) { return f % N::HL; }, A, [](int f) { return f + 4 * (f / N::HL); }, tid);
     __syncthreads();
     if (tid < N::NOUT) {
         const float4* p4 = reinterpret_cast<const float4*>(A + tid * OS);
