@@ -1,11 +1,16 @@
 """Benchmark of the MI355X compact-CNN-cascade detection path.
 
 Default workload (one GPU): 64 seeded 1920x1080 BGR frames, scale factor 1.15, faces
-40..240 px, default FaceDetector parameters, one Detect call per frame (single-frame
-launches), frames resident in GPU memory before the timed region. A step is one pass
-over the 64 frames. Prints one JSON line.
+40..240 px, default FaceDetector parameters, frames resident in GPU memory before the
+timed region. A step is one DetectBatch pass over the 64 frames (per-frame launches,
+consecutive frames pipelined over three streams); `value` is frames/s over K steps.
+Beside it the line reports one synchronous Detect at a time, per-kernel rooflines from a
+single-frame pass, the 4K / mixed-size / 640x480 / deep-pyramid / facial-analysis legs,
+the headline frames handed over in host memory (PCIe upload included) and the
+reference's CPU path on a bounded sample. Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--frames F] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--frames F] [--gpus 1] [--facial]
+                    [--no-secondary] [--no-cpu-baseline]
 """
 import argparse
 import ctypes
@@ -71,6 +76,32 @@ def secondary_fps(api, torch, names, per_name, steps, facial=False):
     return {"fps": round(len(frames) / best, 2), "frames": len(frames), "faces": faces,
             "sizes": sorted({"%dx%d" % (f.shape[1], f.shape[0]) for f in frames}),
             "min_face": cfg["min_face"], "scale_factor": cfg["scale_factor"]}
+
+
+def host_frame_fps(api, torch, frames, param):
+    """Frames/s with the frames in host memory, the PCIe upload included (not part of
+    `value`): pageable and pinned buffers, through DetectBatch (best of 3 passes) and one
+    synchronous Detect at a time."""
+    det = api.FaceDetector()
+    if det.Init(param) != 0:
+        raise SystemExit("FaceDetector.Init failed")
+    pageable = [np.ascontiguousarray(f) for f in frames]
+    pinned = [t.numpy() for t in (torch.from_numpy(f).pin_memory() for f in frames)]
+    out = {}
+    for name, imgs in (("pageable", pageable), ("pinned", pinned)):
+        det.DetectBatch(images=imgs)
+        best = None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            det.DetectBatch(images=imgs)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        t0 = time.perf_counter()
+        for im in imgs:
+            det.Detect(im)
+        out[name] = {"batch_fps": round(len(imgs) / best, 1),
+                     "sync_fps": round(len(imgs) / (time.perf_counter() - t0), 1)}
+    return out
 
 
 def stage1_macs(levels):
@@ -309,6 +340,8 @@ def main():
         line["deep_4800x2400"] = secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2)
         # the headline workload with facial analysis (landmarks network on accepted windows)
         line["fhd_facial"] = secondary_fps(api, torch, ["fhd_min40_max240"], 64, 3, facial=True)
+        # the headline frames handed over in host memory: the upload crosses PCIe
+        line["fhd_host_frames"] = host_frame_fps(api, torch, frames, param)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(frames, dict(min_face=cfg["min_face"], max_face=cfg["max_face"],
                                        scale_factor=cfg["scale_factor"]), args.cpu_seconds)
