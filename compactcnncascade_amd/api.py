@@ -21,6 +21,8 @@ _F = ctypes.c_float
 PIPELINE_CPU, PIPELINE_GPU, PIPELINE_GPU_CPU = 0, 1, 2
 DETECT_DISABLE, DETECT_SYNC, DETECT_ASYNC = 0, 1, 2
 PRECISION_DEF, PRECISION_LOW, PRECISION_NORMAL, PRECISION_HIGH, PRECISION_ULTRA = 0, 1, 2, 3, 4
+# compute precisions of this library (ccnn_set_compute_precision)
+FP32, FP16_BF16, FP16_FP8 = 0, 1, 2
 
 
 class CParam(ctypes.Structure):
@@ -83,6 +85,18 @@ def load():
     L.ccnn_stage_check.argtypes = [ctypes.POINTER(CParam), _P, _I, _I, _I, _P, _P, _P, _P]
     L.ccnn_last_trace.argtypes = [_P, ctypes.POINTER(_I), _P, _I, _P, _P]
     L.ccnn_last_facial.argtypes = [_P, _P, _I]
+    L.ccnn_last_responses.argtypes = [_P, _P, ctypes.c_size_t]
+    L.ccnn_last_responses.restype = ctypes.c_long
+    L.ccnn_stage23.argtypes = [_I, _P, _I, _P]
+    L.ccnn_set_batch_mode.argtypes = [_P, _I]
+    L.ccnn_batch_stats.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]
+    L.ccnn_set_tile_shape.argtypes = [_P, _I]
+    L.ccnn_set_compute_precision.argtypes = [_P, _I]
+    L.ccnn_get_compute_precision.argtypes = [_P]
+    L.ccnn_stage1_ex.argtypes = [_P, _I, _I, _I, _P]
+    L.ccnn_stage23_ex.argtypes = [_I, _P, _I, _I, _P]
+    L.ccnn_get_tile_shape.argtypes = [_P] + [ctypes.POINTER(_I)] * 4
+    L.ccnn_autotune.argtypes = [_P, _I, _P, _I, _I, _P, _P]
     L.ccnn_stage_landmarks.argtypes = [_P, _P]
     L.ccnn_set_profiling.argtypes = [_P, _I]
     L.ccnn_last_error.argtypes = [_I]
@@ -245,6 +259,54 @@ class FaceDetector:
             raise RuntimeError("ccnn_detect_batch failed")
         return [(int(counts[i]), faces[i * capacity:i * capacity + min(int(counts[i]), capacity)]) for i in range(n)]
 
+    def set_batch_mode(self, mode):
+        """1: DetectBatch runs each batch as one graph launch (default); 0: per-frame
+        launches pipelined over three streams."""
+        if self._L.ccnn_set_batch_mode(self._h, int(mode)) != 0:
+            raise ValueError("bad batch mode")
+
+    def set_compute_precision(self, p):
+        """FP32 (exact, default), FP16_BF16 or FP16_FP8."""
+        if self._L.ccnn_set_compute_precision(self._h, int(p)) != 0:
+            raise ValueError("bad compute precision")
+
+    def get_compute_precision(self):
+        return self._L.ccnn_get_compute_precision(self._h)
+
+    def set_tile_shape(self, shape):
+        """Stage-1 tile shape id (0 .. 11) of the frames planned from now on."""
+        if self._L.ccnn_set_tile_shape(self._h, int(shape)) != 0:
+            raise ValueError("bad tile shape")
+
+    def get_tile_shape(self):
+        """(id, (ax, ay, tx, ty))"""
+        v = [_I() for _ in range(4)]
+        i = self._L.ccnn_get_tile_shape(self._h, *[ctypes.byref(x) for x in v])
+        return i, tuple(x.value for x in v)
+
+    def autotune(self, images=None, device_frames=None, reps=3):
+        """Autotunes the batch path on these frames: returns (chosen shape id, per-shape
+        best ms [12], per-grid best ms [3] for 2/3/4 check workgroups per CU)."""
+        if images is not None:
+            keep = [_image(im) for im in images]
+            cimgs = (CImage * len(keep))(*[ci for _, ci in keep])
+            device = 0
+        else:
+            cimgs = (CImage * len(device_frames))(*[CImage(c, r, ch, p, st) for (p, c, r, ch, st) in device_frames])
+            device = 1
+        sms = np.full(12, np.nan, np.float32)
+        gms = np.full(3, np.nan, np.float32)
+        best = self._L.ccnn_autotune(self._h, len(cimgs), cimgs, device, int(reps), _ptr(sms), _ptr(gms))
+        if best < 0:
+            raise RuntimeError("ccnn_autotune failed")
+        return best, sms, gms
+
+    def batch_stats(self):
+        """(kernel launches in the last batch graph, graph captures so far)"""
+        a, b = _I(), _I()
+        self._L.ccnn_batch_stats(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
     def Clear(self):
         return self._L.ccnn_clear(self._h)
 
@@ -332,6 +394,23 @@ class FaceDetector:
         return dict(levels=levels[:nl.value], cand=cand[:n], res=res[:n])
 
 
+    def last_responses(self):
+        """Stage-1 response maps of the frame last_trace describes: a list of float32
+        arrays [rh, rw], one per level in last_trace's level order."""
+        tr = self.last_trace()
+        n = self._L.ccnn_last_responses(self._h, None, 0)
+        if n < 0:
+            raise RuntimeError("no response maps recorded")
+        buf = np.zeros(max(1, n), np.float32)
+        if self._L.ccnn_last_responses(self._h, _ptr(buf), n) != n:
+            raise RuntimeError("ccnn_last_responses failed")
+        out, off = [], 0
+        for (_, _, _, rw, rh, _) in tr["levels"]:
+            out.append(buf[off:off + rw * rh].reshape(rh, rw).copy())
+            off += rw * rh
+        return out
+
+
 def last_error(reset=True):
     """Last HIP error code met by the library on this thread (0 if none)."""
     return load().ccnn_last_error(int(reset))
@@ -355,7 +434,7 @@ def stage_level(gray, lw, lh, mode):
     return out
 
 
-def stage1(level):
+def stage1(level, precision=FP32):
     level = np.ascontiguousarray(level, np.float32)
     lh, lw = level.shape
     rw, rh = _I(), _I()
@@ -363,7 +442,11 @@ def stage1(level):
     if L.ccnn_stage1_size(lw, lh, ctypes.byref(rw), ctypes.byref(rh)) != 0:
         raise ValueError("level too small")
     out = np.zeros((rh.value, rw.value), np.float32)
-    if L.ccnn_stage1(_ptr(level), lw, lh, _ptr(out)) != 0:
+    if precision == FP32:
+        rc = L.ccnn_stage1(_ptr(level), lw, lh, _ptr(out))
+    else:
+        rc = L.ccnn_stage1_ex(_ptr(level), lw, lh, int(precision), _ptr(out))
+    if rc != 0:
         raise RuntimeError("ccnn_stage1 failed")
     return out
 
@@ -392,3 +475,13 @@ def stage_check(param, gray, scales, xs, ys):
                                _ptr(res)) != 0:
         raise RuntimeError("ccnn_stage_check failed")
     return res[:n]
+
+
+def stage23(stage, inputs, precision=FP32):
+    """The stage-2 (stage=1) or stage-3 (stage=2) network on float32 inputs [n, 51, 47]:
+    their 4x4 response maps [n, 4, 4], in a compute precision of this library."""
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 51, 47)
+    out = np.zeros((max(1, len(x)), 4, 4), np.float32)
+    if load().ccnn_stage23_ex(stage, _ptr(x), len(x), int(precision), _ptr(out)) != 0:
+        raise RuntimeError("ccnn_stage23_ex failed")
+    return out[:len(x)]

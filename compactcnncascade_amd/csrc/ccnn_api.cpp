@@ -113,6 +113,70 @@ int ccnn_detect_batch(ccnn_detector* d, int n, const ccnn_image* imgs, int devic
     return 0;
 }
 
+int ccnn_set_batch_mode(ccnn_detector* d, int mode)
+{
+    if (!d || mode < 0 || mode > 1) return -1;
+    d->det.set_batch_mode(mode);
+    return 0;
+}
+
+int ccnn_set_compute_precision(ccnn_detector* d, int precision)
+{
+    if (!d) return -1;
+    return d->det.set_precision(precision);
+}
+
+int ccnn_get_compute_precision(const ccnn_detector* d)
+{
+    return d ? d->det.precision() : -1;
+}
+
+int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int min_face, float scale_factor)
+{
+    if (!d) return -1;
+    return d->det.set_skipnet(params, threshold, min_face, scale_factor);
+}
+
+int ccnn_skip_stats(const ccnn_detector* d, long long* tiles, long long* kept)
+{
+    if (!d || !tiles || !kept) return -1;
+    d->det.skip_stats(tiles, kept);
+    return 0;
+}
+
+int ccnn_set_tile_shape(ccnn_detector* d, int shape)
+{
+    if (!d) return -1;
+    return d->det.set_tile_shape(shape);
+}
+
+int ccnn_get_tile_shape(const ccnn_detector* d, int* ax, int* ay, int* tx, int* ty)
+{
+    if (!d) return -1;
+    const int id = d->det.tile_shape_id();
+    const ccnn::TileShape t = ccnn::tile_shape(id);
+    if (ax) *ax = t.ax;
+    if (ay) *ay = t.ay;
+    if (tx) *tx = t.tx;
+    if (ty) *ty = t.ty;
+    return id;
+}
+
+int ccnn_autotune(ccnn_detector* d, int n, const ccnn_image* imgs, int device, int reps, float* shape_ms,
+                  float* grid_ms)
+{
+    if (!d || d->det.empty() || n <= 0 || !imgs || reps <= 0) return -1;
+    return d->det.autotune(n, imgs, device != 0, reps, shape_ms, grid_ms);
+}
+
+int ccnn_batch_stats(const ccnn_detector* d, int* launches, int* captures)
+{
+    if (!d) return -1;
+    if (launches) *launches = d->det.batch_launches();
+    if (captures) *captures = d->det.batch_captures();
+    return 0;
+}
+
 int ccnn_clear(ccnn_detector* d)
 {
     if (!d || d->det.empty()) return -1;
@@ -152,14 +216,15 @@ int ccnn_get_max_image_size(const ccnn_detector* d, int* w, int* h)
 }
 
 // The setters change one parameter and re-initialise lazily, as CNNDetector's setters
-// Clear() and the next Detect re-Inits (cnn_detector_v3.h:404-436, cnn_detector_v3.cpp:2485-2492).
+// Clear() and the next Detect re-Inits (cnn_detector_v3.h:404-436, cnn_detector_v3.cpp:2485-2492);
+// gray_image_only keeps its state (only Init and setParam force it on).
 void ccnn_set_max_image_size(ccnn_detector* d, int w, int h)
 {
     if (!d || d->det.empty()) { std::printf("[CompactCNNLibAPI] FaceDetector no initialized!\n"); return; }
     ccnn_param p = d->det.param;
     p.max_image_width = w;
     p.max_image_height = h;
-    d->det.init(p);
+    d->det.set_fields(p);
 }
 
 int ccnn_get_min_object_height(const ccnn_detector* d)
@@ -173,7 +238,7 @@ void ccnn_set_min_object_height(ccnn_detector* d, int v)
     if (!d || d->det.empty()) { std::printf("[CompactCNNLibAPI] FaceDetector no initialized!\n"); return; }
     ccnn_param p = d->det.param;
     p.min_face_height = v;
-    d->det.init(p);
+    d->det.set_fields(p);
 }
 
 int ccnn_get_max_object_height(const ccnn_detector* d)
@@ -187,7 +252,7 @@ void ccnn_set_max_object_height(ccnn_detector* d, int v)
     if (!d || d->det.empty()) { std::printf("[CompactCNNLibAPI] FaceDetector no initialized!\n"); return; }
     ccnn_param p = d->det.param;
     p.max_face_height = v;
-    d->det.init(p);
+    d->det.set_fields(p);
 }
 
 int ccnn_get_num_threads(const ccnn_detector* d)
@@ -224,6 +289,12 @@ int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
 {
     if (!d) return -1;
     return d->det.last_trace(n_levels, levels, cap, cand, res);
+}
+
+long ccnn_last_responses(ccnn_detector* d, float* out, size_t cap)
+{
+    if (!d) return -1;
+    return d->det.last_responses(out, cap);
 }
 
 int ccnn_last_facial(const ccnn_detector* d, int* out, int cap)

@@ -3,6 +3,7 @@
 #include "ccnn_detector.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,6 +23,11 @@ thread_local int g_last_error = 0;
             return -1;                                                                       \
         }                                                                                    \
     } while (0)
+
+void note_error(hipError_t e)
+{
+    if (e != hipSuccess) g_last_error = (int)e;
+}
 
 int last_error(bool reset)
 {
@@ -51,6 +57,92 @@ struct Reader {
 };
 
 }  // namespace
+
+// Two FP32 values rounded to FP16 (to nearest even) and packed into one word, low first.
+static uint32_t pack_half2(float a, float b)
+{
+    const _Float16 x = (_Float16)a, y = (_Float16)b;
+    uint16_t ux, uy;
+    std::memcpy(&ux, &x, 2);
+    std::memcpy(&uy, &y, 2);
+    return (uint32_t)ux | ((uint32_t)uy << 16);
+}
+
+void append_stage1_half(std::vector<float>& w)
+{
+    using N = Stage1;
+    std::vector<uint32_t> h(Stage1H::SIZE, 0u);
+    for (int m = 0; m < N::M1; ++m)
+        for (int r = 0; r < 4; ++r) {
+            const float* k = &w[N::oK1 + m * 16 + r * 4];
+            h[Stage1H::oL1 + m * 8 + 2 * r] = pack_half2(k[0], k[1]);
+            h[Stage1H::oL1 + m * 8 + 2 * r + 1] = pack_half2(k[2], k[3]);
+        }
+    for (int m = 0; m < N::M2; ++m)
+        for (int r = 0; r < 3; ++r) {
+            const float* k = &w[N::oK2 + m * 9 + r * 3];
+            h[Stage1H::oL2 + m * 6 + 2 * r] = pack_half2(k[0], k[1]);
+            h[Stage1H::oL2 + m * 6 + 2 * r + 1] = pack_half2(k[2], 0.f);
+        }
+    for (int m = 0; m < N::M3; ++m)
+        for (int r = 0; r < N::K3H; ++r) {
+            const float* k = &w[N::oK3 + (m * N::K3H + r) * N::K3W];
+            h[Stage1H::oL3 + (m * N::K3H + r) * 2] = pack_half2(k[0], k[1]);
+            h[Stage1H::oL3 + (m * N::K3H + r) * 2 + 1] = pack_half2(k[2], k[3]);
+        }
+    for (int u = 0; u < N::HL; ++u) {
+        const float* k = &w[N::oHW + u * N::HLN];
+        h[Stage1H::oHL + 2 * u] = pack_half2(k[0], k[1]);
+        h[Stage1H::oHL + 2 * u + 1] = pack_half2(k[2], k[3]);
+    }
+    const size_t base = w.size();
+    w.resize(base + Stage1H::SIZE);
+    std::memcpy(&w[base], h.data(), sizeof(uint32_t) * h.size());
+}
+
+// 2^(8 - E) for the smallest power of two 2^E above m: the power of two that brings m
+// into [128, 256) (FP8 e4m3 operands reach 448).
+static float fp8_scale(float m)
+{
+    if (!(m > 0.f)) return 1.f;
+    int e;
+    std::frexp(m, &e);
+    return std::ldexp(1.f, 8 - e);
+}
+
+void mfma_weights(const std::vector<float>& w, std::vector<float>& out)
+{
+    using N = Stage2;              // the stage-3 convolutions have the same shape
+    static_assert(Stage3::oK1 == N::oK1 && Stage3::oK2 == N::oK2 && Stage3::M1 == N::M1 && Stage3::M2 == N::M2,
+                  "one layout for both check networks");
+    out.assign(MfmaW::SIZE, 0.f);
+    float m1 = 0.f, m2 = 0.f;
+    for (int nt = 0; nt < 2; ++nt)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 8; ++j) {
+                const int col = (lane & 15) + 16 * nt, m = col >> 2, q = col & 3, t = 8 * (lane >> 4) + j;
+                if (m >= N::M1 || t >= 25) continue;
+                const int rr = t / 5 - (q >> 1), cc = t % 5 - (q & 1);
+                if (rr < 0 || rr > 3 || cc < 0 || cc > 3) continue;
+                const float v = w[N::oK1 + m * 16 + rr * 4 + cc];
+                out[MfmaW::oL1 + (nt * 64 + lane) * 8 + j] = v;
+                m1 = std::max(m1, std::fabs(v));
+            }
+    for (int jp = 0; jp < 3; ++jp)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 8; ++j) {
+                const int col = lane & 15, kbk = lane >> 4, side = kbk >> 1, t = (kbk & 1) * 8 + j;
+                if ((side == 0) != (col < 8)) continue;
+                const int m = 4 * jp + (col >> 2), q = col & 3;
+                const int rr = (t >> 2) - (q >> 1), cc = (t & 3) - (q & 1);
+                if (rr < 0 || rr > 2 || cc < 0 || cc > 2) continue;
+                const float v = w[N::oK2 + m * 9 + rr * 3 + cc];
+                out[MfmaW::oL2 + (jp * 64 + lane) * 8 + j] = v;
+                m2 = std::max(m2, std::fabs(v));
+            }
+    out[MfmaW::oScale] = fp8_scale(m1);
+    out[MfmaW::oScale + 1] = fp8_scale(m2);
+}
 
 int load_model(const unsigned char* bytes, size_t n, int stage, int index_output, NetWeights& out, std::string& err)
 {
@@ -150,26 +242,33 @@ int load_model(const unsigned char* bytes, size_t n, int stage, int index_output
         w[N::oOB + N::NOUT + 1] = af_scale;
         return 0;
     };
-    if (stage == 0) return fill(Stage1{});
+    if (stage == 0) {
+        if (fill(Stage1{}) < 0) return -1;
+        append_stage1_half(out.w);
+        return 0;
+    }
     if (stage == 1) return fill(Stage2{});
     if (stage == 2) return fill(Stage3{});
     return fill(Landmarks{});
 }
 
-size_t level_l1_layout(LevelDesc& L)
+size_t level_l1_layout(LevelDesc& L, int shape)
 {
+    const TileShape t = tile_shape(shape);
+    L.tiles_x = L.rw > 0 ? (L.rw + t.tx - 1) / t.tx : 0;
+    L.tiles_y = L.rh > 0 ? (L.rh + t.ty - 1) / t.ty : 0;
     L.s1_w = L.s1_h = 0;
     L.l1_tiles_x = L.l1_tiles_y = 0;
     if (L.tiles_x <= 0 || L.tiles_y <= 0) return 0;
     // the response tiles read pooled L1 rows 2*oy0 .. 2*oy0 + 2*TY + 9 and columns
     // 2*ox0 .. 2*ox0 + 2*TX + 7
-    L.s1_w = 2 * S1_TX * L.tiles_x + 8;
-    L.s1_h = 2 * S1_TY * L.tiles_y + 10;
+    L.s1_w = 2 * t.tx * L.tiles_x + 8;
+    L.s1_h = 2 * t.ty * L.tiles_y + 10;
     // the L1 tiles cover the positions that responses inside the map read: columns
     // < 2*rw + 8, rows < 2*rh + 10. The rest of the planes (read only by the tiles'
     // threads past the response map, whose results are dropped) is never written.
-    L.l1_tiles_x = (2 * L.rw + 8 + S1A_TX - 1) / S1A_TX;
-    L.l1_tiles_y = (2 * L.rh + 10 + S1A_TY - 1) / S1A_TY;
+    L.l1_tiles_x = (2 * L.rw + 8 + t.ax - 1) / t.ax;
+    L.l1_tiles_y = (2 * L.rh + 10 + t.ay - 1) / t.ay;
     return (size_t)Stage1::M1 * L.s1_w * L.s1_h;
 }
 
@@ -353,34 +452,6 @@ std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s)
 // ---------------------------------------------------------------------------------
 namespace {
 
-template <class T>
-void dfree(T*& p)
-{
-    if (p) { (void)hipFree((void*)p); p = nullptr; }
-}
-template <class T>
-void hfree(T*& p)
-{
-    if (p) { (void)hipHostFree((void*)p); p = nullptr; }
-}
-template <class T>
-int dalloc(T*& p, size_t n)
-{
-    dfree(p);
-    return hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1)) == hipSuccess ? 0 : -1;
-}
-// pinned host memory mapped into the device address space; dp receives the device view
-template <class T>
-int halloc(T*& p, T*& dp, size_t n)
-{
-    hfree(p);
-    dp = nullptr;
-    if (hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocMapped | hipHostMallocCoherent) !=
-        hipSuccess)
-        return -1;
-    return hipHostGetDevicePointer((void**)&dp, p, 0) == hipSuccess ? 0 : -1;
-}
-
 // the pinned read-back buffers hold at least this many candidates
 constexpr size_t PREFETCH = 4096;
 // and at most this many; larger candidate lists are copied to pageable memory
@@ -395,8 +466,11 @@ Detector::Detector()
 
 Detector::~Detector()
 {
+    release_batch();
     for (auto& s : slots_) release_slot(s);
     for (auto& w : d_w_) dfree(w);
+    for (auto& w : d_mw_) dfree(w);
+    dfree(d_skip_);
     for (Plan* p : plans_) delete p;
 }
 
@@ -432,9 +506,11 @@ int Detector::init_device()
         for (auto& e : s.kev)
             if (!e) HIP_OK(hipEventCreate(&e));
     }
-    if (param.device_info) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, device_) == hipSuccess)
+    if (!batch_.st) HIP_OK(hipStreamCreateWithFlags(&batch_.st, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_) == hipSuccess) {
+        cus_ = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : cus_;
+        if (param.device_info)
             std::printf("[ccnn] device %d: %s (%s), %d CUs\n", device_, prop.name, prop.gcnArchName,
                         prop.multiProcessorCount);
     }
@@ -470,10 +546,18 @@ int Detector::build()
         }
         if (dalloc(d_w_[st], net_[st].w.size()) < 0) return -1;
         HIP_OK(hipMemcpy(d_w_[st], net_[st].w.data(), sizeof(float) * net_[st].w.size(), hipMemcpyHostToDevice));
+        if (st == 1 || st == 2) {
+            // the matrix-core form of the check networks (reduced precision)
+            std::vector<float> mw;
+            mfma_weights(net_[st].w, mw);
+            if (dalloc(d_mw_[st - 1], mw.size()) < 0) return -1;
+            HIP_OK(hipMemcpy(d_mw_[st - 1], mw.data(), sizeof(float) * mw.size(), hipMemcpyHostToDevice));
+        }
     }
     float sf;
     scales_ = init_scales(param, net_[0].min_h, gray_only_, &pre_scale_, &sf);
     param.scale_factor = sf;                                     // :325-326 writes it back
+    num_scales_ = (int)scales_.size();                           // :603
     if (scales_.empty()) {
         std::printf("[CNNDetector] Invalid configuration!\n");
         return -1;
@@ -493,16 +577,28 @@ int Detector::build()
     for (Plan* pl : plans_) delete pl;
     plans_.clear();
     for (auto& s : slots_) s.levels_plan = nullptr;
+    batch_.key.clear();          // the plans the batch graph was built from are gone
+    batch_.gen++;
     built_ = true;
     empty_ = false;
     return 0;
 }
 
+// FaceDetector::Clear deletes the detector (CompactCNNLibAPI_v2.cpp:355-372): every device
+// and pinned buffer, the weights and the cached plans are released; Init builds them again.
 int Detector::clear()
 {
     if (empty_) return -1;
-    for (auto& s : slots_)
-        if (s.busy) { (void)hipStreamSynchronize(s.st); s.busy = false; }
+    release_batch();
+    for (auto& s : slots_) release_slot(s);
+    for (auto& w : d_w_) dfree(w);
+    for (auto& w : d_mw_) dfree(w);
+    for (Plan* p : plans_) delete p;
+    plans_.clear();
+    last_slot_ = -1;
+    last_gray_ = nullptr;
+    last_resp_ = nullptr;
+    std::vector<float>().swap(h_gray_);
     empty_ = true;
     built_ = false;
     return 0;
@@ -511,21 +607,30 @@ int Detector::clear()
 // Level table for a gray image of gw x gh: levels scl_max -> 0 (RunCPUDetect order,
 // which is also the RunCheckDetectAsync order), skip rule (:2083-2088), resize mode
 // (:2093-2104), response size (getOutputImgSize, cnn_simd_cntk.cpp:997-1012).
-const Plan* Detector::plan_for(int gw, int gh)
+const Plan* Detector::plan_for(int gw, int gh, int ncap)
 {
+    ncap = std::min(std::max(ncap, 0), (int)scales_.size());
+    const int shape = shape_;
     for (const Plan* p : plans_)
-        if (p->gw == gw && p->gh == gh) return p;
+        if (p->gw == gw && p->gh == gh && p->ncap == ncap && p->shape == shape) return p;
     Plan* P = new Plan();
+    P->id = ++plan_ids_;
+    P->shape = shape;
     P->gw = gw;
     P->gh = gh;
+    P->ncap = ncap;
+    P->next_num_scales = ncap;
     int tiles = 0, chunks = 0, l1tiles = 0;
     size_t roff = 0, s1off = 0;
-    for (int scl = (int)scales_.size() - 1; scl >= 0; --scl) {
+    for (int scl = ncap - 1; scl >= 0; --scl) {
         LevelDesc L{};
         L.scale = scales_[scl];
         L.scl = scl;
         size_mul(gw, gh, L.scale, &L.lw, &L.lh);
-        if (L.lw < ext_w_ || L.lh < ext_h_) continue;
+        if (L.lw < ext_w_ || L.lh < ext_h_) {
+            P->next_num_scales = std::min(P->next_num_scales, scl);
+            continue;
+        }
         L.mode = (L.scale != 1.f) ? (L.scale > 0.7f ? 0 : 1) : 2;
         if (L.lw == gw && L.lh == gh) L.mode = 2;       // FastImageResize copies equal sizes
         L.sclx = ((uint32_t)gw << 12) / (uint32_t)L.lw + 1;
@@ -537,12 +642,10 @@ const Plan* Detector::plan_for(int gw, int gh)
         L.resp_off = (int)roff;
         roff += (size_t)L.rw * L.rh;
         L.tile_off = tiles;
-        L.tiles_x = (L.rw + S1_TX - 1) / S1_TX;
-        L.tiles_y = (L.rh + S1_TY - 1) / S1_TY;
-        tiles += L.tiles_x * L.tiles_y;
         L.s1_off = (int)s1off;
         L.l1_tile_off = l1tiles;
-        s1off += level_l1_layout(L);
+        s1off += level_l1_layout(L, shape);
+        tiles += L.tiles_x * L.tiles_y;
         l1tiles += L.l1_tiles_x * L.l1_tiles_y;
         L.chunk_off = chunks;
         L.chunks = (L.rw * L.rh + CMP_CHUNK - 1) / CMP_CHUNK;
@@ -569,7 +672,7 @@ const Plan* Detector::plan_for(int gw, int gh)
         }
     if (plans_.size() >= 8) {          // a handful of frame sizes stay cached
         for (size_t k = 0; k < plans_.size(); ++k) {
-            bool used = false;
+            bool used = plans_[k]->pins > 0;
             for (auto& s : slots_) used |= s.busy && s.plan == plans_[k];
             if (used) continue;
             for (auto& s : slots_)
@@ -655,6 +758,8 @@ int Detector::prepare(const ccnn_image& img)
             std::printf("[CNNDetector] CNN detector no initialized!\n");
             return -1;
         }
+    } else if (w != param.max_image_width || h != param.max_image_height) {
+        num_scales_ = (int)scales_.size();                                   // :2514-2518
     }
     if (gray_only_ && pre_scale_ > 0.f && ch != 1) {                          // :2540-2546
         std::printf("[CNNDetector] gray_image_only flag set to false!\n");
@@ -711,15 +816,31 @@ CheckParams Detector::check_params() const
 // Launches one frame on slot s: upload (host frames), optional 8-bit pre-resize, gray,
 // stage 1 over all levels (which appends the candidate windows), stage-2/3 checks,
 // asynchronous read-back.
+const Plan* Detector::plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh)
+{
+    *gw = img.cols;
+    *gh = img.rows;
+    *pre = gray_only_ && pre_scale_ > 0.f && img.channels == 1;
+    if (*pre) size_mul(img.cols, img.rows, pre_scale_, gw, gh);
+    const Plan* plan = plan_for(*gw, *gh, num_scales_);
+    num_scales_ = plan->next_num_scales;
+    return plan;
+}
+
 int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
+{
+    bool pre;
+    int gw, gh;
+    const Plan* plan = plan_frame(img, &pre, &gw, &gh);
+    return enqueue_plan(s, img, device_ptr, plan, pre, gw, gh);
+}
+
+int Detector::enqueue_plan(Slot& s, const ccnn_image& img, bool device_ptr, const Plan* plan, bool pre, int gw, int gh)
 {
     HIP_OK(hipSetDevice(device_));
     const int w = img.cols, h = img.rows, ch = img.channels;
-    int gw = w, gh = h, gch = ch;
+    int gch = ch;
     size_t gstep = img.step;
-    const bool pre = gray_only_ && pre_scale_ > 0.f && ch == 1;
-    if (pre) size_mul(w, h, pre_scale_, &gw, &gh);
-    const Plan* plan = plan_for(gw, gh);
     const size_t bytes = img.step * (size_t)(h - 1) + (size_t)w * ch;
     if (ensure_slot(s, std::max(w, gw), std::max(h, gh), device_ptr ? 0 : bytes, *plan) < 0) return -1;
     hipStream_t st = s.st;
@@ -757,7 +878,7 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
     s.host_decide = s.checks;
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
                          s.d_resp, set_.t1, s.d_meta, s.d_cands, s.host_decide ? s.hd_cands : nullptr, (int)s.pre,
-                         s.d_l1_order, st, profiling_ ? s.kev : nullptr));
+                         s.d_l1_order, st, profiling_ ? s.kev : nullptr, plan->shape, precision_ > 0));
     if (plan->ntiles <= 0) {
         HIP_OK(hipMemsetAsync(s.d_meta, 0, sizeof(int), st));
         if (profiling_)
@@ -769,7 +890,8 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
                             d_w_[2], s.d_s2, s.d_s3, s.d_results, check_blocks_,
                             s.host_decide ? s.hd_meta : nullptr, nullptr, nullptr,
                             s.host_decide ? s.hd_s23 : nullptr, s.host_decide ? s.hd_s23 + 2 * s.h_cap : nullptr,
-                            (int)s.pre, st, profiling_ ? s.kev + 4 : nullptr));
+                            (int)s.pre, st, profiling_ ? s.kev + 4 : nullptr, 0x7fffffff, nullptr, precision_,
+                            d_mw_[0], d_mw_[1]));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), s.d_s2,
                                  s.d_s3, d_w_[3], s.d_results, s.hd_results, (int)s.pre, fa_blocks_, st));
@@ -885,11 +1007,24 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
     }
     last_count_ = count;
     last_levels_ = levels;
+    last_gray_ = s.d_gray;
+    last_gp_ = s.gpitch;
+    last_gw_ = s.gw;
+    last_gh_ = s.gh;
+    last_resp_ = s.d_resp;
+    const std::vector<Det> rects = replay(levels.data(), nl, cands, res, count, checks);
+    emit_faces(rects, s.rescale, faces);
+    return (int)faces.size();
+}
 
-    // Ordered replay of RunCheckDetect / CPUCheckDetect (cnn_detector_v3.cpp:1842-2011,
-    // 1485-1818): the scan-time DropDetection against all detections so far, then per
-    // surviving point the drop against this level's detections and the precomputed
-    // stage-2/3 outcome.
+// Ordered replay of RunCheckDetect / CPUCheckDetect (cnn_detector_v3.cpp:1842-2011,
+// 1485-1818) over one frame's candidates in check order (level, then raster): the
+// scan-time DropDetection against all detections so far, then per surviving point the
+// drop against this level's detections and the precomputed stage-2/3 outcome. Returns
+// the detections before post-processing.
+std::vector<Det> Detector::replay(const LevelDesc* levels, int nl, const Candidate* cands, const CheckResult* res,
+                                  int count, bool checks) const
+{
     std::vector<Det> rects;
     std::vector<int> points;
     int ci = 0;
@@ -967,12 +1102,21 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
             }
         }
     }
+    return rects;
+}
+
+// Post-processing of Detect and FaceDetector's output conversion: Merger passes, the
+// rescale of a pre-resized frame's boxes (cnn_detector_v3.cpp:2783-2806) and the facial
+// attributes per face.
+void Detector::emit_faces(const std::vector<Det>& rects, bool rescale, std::vector<ccnn_face>& faces) const
+{
+    faces.clear();
     std::vector<Det> dets = post_process(rects, set_);
-    const float sc = s.rescale ? (float)param.min_face_height / (float)net_[0].min_h : 1.f;   // :2783-2806
+    const float sc = rescale ? (float)param.min_face_height / (float)net_[0].min_h : 1.f;   // :2783-2806
     faces.reserve(dets.size());
     for (const Det& d : dets) {
         Rect r = d.r;
-        if (s.rescale)
+        if (rescale)
             r = Rect((int)((float)r.x * sc), (int)((float)r.y * sc), (int)((float)r.w * sc), (int)((float)r.h * sc));
         ccnn_face f;
         f.x = r.x; f.y = r.y; f.width = r.w; f.height = r.h; f.score = d.score;
@@ -990,7 +1134,6 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
         }
         faces.push_back(f);
     }
-    return (int)faces.size();
 }
 
 int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces)
@@ -1005,6 +1148,13 @@ int Detector::detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_fa
 }
 
 int Detector::detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces)
+{
+    if (batch_mode_ == 1 && !set_.facial && n > 0) return detect_batch_graph(n, imgs, device_ptr, faces);
+    return detect_batch_pipelined(n, imgs, device_ptr, faces);
+}
+
+int Detector::detect_batch_pipelined(int n, const ccnn_image* imgs, bool device_ptr,
+                                     std::vector<std::vector<ccnn_face>>& faces)
 {
     faces.assign(std::max(n, 0), std::vector<ccnn_face>());
     // frames go to the slots round robin; when every slot holds a frame in flight, the
@@ -1046,21 +1196,20 @@ int Detector::detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::
 
 int Detector::gray_image(ccnn_image* img, int pipeline)
 {
-    if (!img || empty_ || last_slot_ < 0) return -1;
-    Slot& s = slots_[last_slot_];
-    if (!s.d_gray || s.gw <= 0) return -1;
-    if (s.busy) (void)hipStreamSynchronize(s.st);
+    if (!img || empty_ || !last_gray_ || last_gw_ <= 0) return -1;
+    for (auto& s : slots_)
+        if (s.busy) (void)hipStreamSynchronize(s.st);
     if (pipeline == CCNN_PIPELINE_CPU) {
-        h_gray_.resize((size_t)s.gpitch * s.gh);
-        HIP_OK(hipMemcpy(h_gray_.data(), s.d_gray, sizeof(float) * h_gray_.size(), hipMemcpyDeviceToHost));
+        h_gray_.resize((size_t)last_gp_ * last_gh_);
+        HIP_OK(hipMemcpy(h_gray_.data(), last_gray_, sizeof(float) * h_gray_.size(), hipMemcpyDeviceToHost));
         img->data = (unsigned char*)h_gray_.data();
     } else {
-        img->data = (unsigned char*)s.d_gray;
+        img->data = (unsigned char*)last_gray_;
     }
-    img->cols = s.gw;
-    img->rows = s.gh;
+    img->cols = last_gw_;
+    img->rows = last_gh_;
     img->channels = 1;
-    img->step = (size_t)s.gpitch;
+    img->step = (size_t)last_gp_;
     return 0;
 }
 
@@ -1094,6 +1243,19 @@ int Detector::last_trace(int* n_levels, int* levels, int cap, float* cand, float
         o[4] = r.m1; o[5] = r.m2; o[6] = (float)r.n2; o[7] = (float)r.n3;
     }
     return last_count_;
+}
+
+long Detector::last_responses(float* out, size_t cap)
+{
+    if (empty_ || !last_resp_) return -1;
+    size_t n = 0;
+    for (const LevelDesc& L : last_levels_) n += (size_t)L.rw * L.rh;
+    if (!out) return (long)n;
+    if (cap < n) return -1;
+    for (auto& s : slots_)
+        if (s.busy) HIP_OK(hipStreamSynchronize(s.st));
+    HIP_OK(hipMemcpy(out, last_resp_, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return (long)n;
 }
 
 int Detector::last_facial(int* out, int cap) const

@@ -21,27 +21,84 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
                          int* meta, Candidate* cands, Candidate* hcands, int hcap, const int2* l1_order,
-                         hipStream_t st, const hipEvent_t* kev = nullptr);
-// L1-sum plane geometry and L1 tiles of a level whose response tiles are set
-// (LevelDesc::tiles_x / tiles_y); returns the floats the level's planes take.
-size_t level_l1_layout(LevelDesc& L);
+                         hipStream_t st, const hipEvent_t* kev = nullptr, int shape = 0, bool half = false,
+                         const unsigned char* keep = nullptr, const unsigned char* l1_keep = nullptr);
+// Response tiles, L1-sum plane geometry and L1 tiles of a level whose response map size
+// is set, for tile shape `shape`; returns the floats the level's planes take.
+size_t level_l1_layout(LevelDesc& L, int shape = 0);
+// cap: candidates stored in cands (the list may count more, then the caller redoes the
+// work); queue (optional): a zeroed counter from which a persistent grid takes its items
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
                         CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st,
-                        const hipEvent_t* kev = nullptr);
+                        const hipEvent_t* kev = nullptr, int cap = 0x7fffffff, int* queue = nullptr, int rp = 0,
+                        const float* MW2 = nullptr, const float* MW3 = nullptr);
+// pre-resize of frame f of a batch (FrameDesc::orig, ow x oh -> src, dw x dh)
+hipError_t launch_resize_u8_tab(const FrameDesc* frames, int f, int dw, int dh, hipStream_t st);
+// the tile-skipping network on every frame of a batch and the keep flags of its stage-1
+// response tiles (keep) and L1 tiles (l1_keep); kept (optional) counts the kept tiles
+hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
+                          const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
+                          int ntiles, int nl1tiles, int shape, float threshold, unsigned char* keep,
+                          unsigned char* l1_keep, int* kept, hipStream_t st);
+// gray of every frame of a batch (FrameDesc table in device memory)
+hipError_t launch_gray_batch(const FrameDesc* frames, int nframes, int total_quads, float* gray, hipStream_t st);
+// ordered compaction of the windows above t1 over n responses (all levels of a batch back
+// to back): counts/offs hold one int per CMP_BLOCK positions; the first cap candidates go
+// to cands (and the first hcap to hcands); meta[0] / hmeta[0] = total, meta[1] = 0
+hipError_t launch_compact(const float* resp, int n, float t1, int* counts, int* offs, const LevelDesc* d_levels,
+                          int nlevels, Candidate* cands, int cap, Candidate* hcands, int hcap, int* meta, int* hmeta,
+                          hipStream_t st);
 // k_facial takes the check decisions from the stage-2/3 outcomes itself and writes the facial
 // results of the accepted candidates to results[].fa (and hresults[].fa below hcap)
 hipError_t launch_facial(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                          const int* meta, const CheckParams& P, const int2* s2res, const int2* s3res, const float* W,
                          CheckResult* results, CheckResult* hresults, int hcap, int nblocks, hipStream_t st);
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st);
+// stage 1 (stage-2 network) or 2 (stage-3 network) on n 47x51 inputs; 16 outputs each.
+// rp: 0 exact FP32, 1 BF16 / 2 FP8 matrix-core L1 and L2 (MW: the network's MfmaW table)
+hipError_t launch_net23(int stage, const float* in, int n, const float* W, float* out, hipStream_t st, int rp = 0,
+                        const float* MW = nullptr);
+// The MfmaW table of a stage-2/3 network (NetShape layout w)
+void mfma_weights(const std::vector<float>& w, std::vector<float>& out);
 hipError_t launch_selftest_tanh(uint32_t lo, uint32_t hi, unsigned long long* out, hipStream_t st);
 hipError_t launch_export(const int* meta, const Candidate* cands, const CheckResult* results, int* hmeta,
                          Candidate* hcands, CheckResult* hresults, int cap, hipStream_t st);
 
 // Last HIP error code seen by the library on this thread (0 if none); optionally reset.
 int last_error(bool reset);
+// Records a HIP error as last_error reports it.
+void note_error(hipError_t e);
+
+// Device and pinned-host allocation helpers (each frees what the pointer held first).
+template <class T>
+void dfree(T*& p)
+{
+    if (p) { (void)hipFree((void*)p); p = nullptr; }
+}
+template <class T>
+void hfree(T*& p)
+{
+    if (p) { (void)hipHostFree((void*)p); p = nullptr; }
+}
+template <class T>
+int dalloc(T*& p, size_t n)
+{
+    dfree(p);
+    return hipMalloc((void**)&p, sizeof(T) * (n > 0 ? n : 1)) == hipSuccess ? 0 : -1;
+}
+// pinned host memory mapped into the device address space; dp receives the device view
+template <class T>
+int halloc(T*& p, T*& dp, size_t n)
+{
+    hfree(p);
+    dp = nullptr;
+    if (hipHostMalloc((void**)&p, sizeof(T) * (n > 0 ? n : 1), hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess)
+        return -1;
+    return hipHostGetDevicePointer((void**)&dp, p, 0) == hipSuccess ? 0 : -1;
+}
 
 // A model of the cascade in the reference's binary format, repacked in NetShape layout.
 struct NetWeights {
@@ -49,11 +106,14 @@ struct NetWeights {
     int min_w = 0, min_h = 0;
 };
 // Parses format v1.0-1.1 (cnn_simd_cntk.cpp:42-404) and checks the shape against the
-// stage's prototype. Returns 0 or -1 (message in err).
+// stage's prototype. Stage 0 (stage 1 of the cascade) also gets its FP16 tap pairs
+// (Stage1H) after the FP32 weights. Returns 0 or -1 (message in err).
 int load_model(const unsigned char* bytes, size_t n, int stage, int index_output, NetWeights& out, std::string& err);
 // Embedded copies of the shipped models (stage 0..2: cnn4face{1,2,3}_cntk.bin,
 // 3: cnn4landmarks_cntk.bin).
 void embedded_model(int stage, const unsigned char** bytes, size_t* n);
+// Appends the Stage1H FP16 tap pairs of the stage-1 weights w (NetShape layout).
+void append_stage1_half(std::vector<float>& w);
 
 // Effective detector settings (FaceDetector::Init's conversion, CompactCNNLibAPI_v2.cpp:100-138,
 // and CNNDetector::Init, cnn_detector_v3.cpp:922-964).
@@ -89,9 +149,19 @@ void merger(std::vector<Det>& out, std::vector<Det>& in, float threshold, bool d
 // Merger 0.5, min_num_detect filter, Merger 0.2 with deletion.
 std::vector<Det> post_process(const std::vector<Det>& raw, const Settings& s);
 
-// Pyramid plan of one gray-image size: levels in compute order (scl_max -> 0).
+// Decision of CPUCheckDetect (cnn_detector_v3.cpp:1709-1725, 1808-1811) from the stage-2/3
+// outcomes of both crop variants of a candidate.
+CheckResult check_decision(const CheckParams& P, float scale, const int2* s2, const int2* s3);
+
+// Pyramid plan of one gray-image size: levels in compute order (scl_max -> 0) among the
+// first ncap scales, and the scale count the detector keeps for its next frame.
 struct Plan {
+    long long id = 0;            // unique per detector (the batch graph's key names plans by it)
+    int shape = 0;               // stage-1 tile shape the levels are laid out for
+    int pins = 0;                // frames of a batch being assembled that use the plan
     int gw = -1, gh = -1;
+    int ncap = 0;                // scales visited (CNNDetector::num_scales on entry)
+    int next_num_scales = 0;     // num_scales after the skip rule (cnn_detector_v3.cpp:2083-2088)
     std::vector<LevelDesc> levels;
     int ntiles = 0, nchunks = 0, nl1tiles = 0;
     size_t resp_total = 0;
@@ -135,6 +205,48 @@ struct Slot {
     size_t pre = 0;
 };
 
+// One frame of a batch, planned (ccnn_batch.cpp).
+struct BatchFrame {
+    int idx;                      // index in the caller's frame list
+    const Plan* plan;
+    const uint8_t* data;          // the caller's frame (host or device memory)
+    int w, h, ch;
+    size_t step;
+    bool pre;                     // gray_image_only pre-resize to gw x gh
+    int gw, gh;
+};
+
+// Device buffers and the captured graph of the batch path: every level of every frame of a
+// batch goes through one graph launch (gray, stage 1, ordered compaction, checks).
+struct BatchArena {
+    hipStream_t st = nullptr;
+    uint8_t* d_frames = nullptr;  size_t frames_cap = 0;   // host frames, uploaded
+    uint8_t* d_pre = nullptr;     size_t pre_cap = 0;      // pre-resized frames
+    float* d_gray = nullptr;      size_t gray_cap = 0;
+    float* d_s1 = nullptr;        size_t s1_cap = 0;
+    float* d_resp = nullptr;      size_t resp_cap = 0;
+    LevelDesc* d_levels = nullptr; size_t levels_cap = 0;
+    int2* d_l1_order = nullptr;   size_t l1_cap = 0;
+    FrameDesc* d_ftab = nullptr;  FrameDesc* h_ftab = nullptr; FrameDesc* hd_ftab = nullptr; size_t ftab_cap = 0;
+    int* d_counts = nullptr;      int* d_offs = nullptr;   size_t cnt_cap = 0;
+    int* d_meta = nullptr;
+    Candidate* d_cands = nullptr; int2* d_s2 = nullptr; int2* d_s3 = nullptr; size_t cand_cap = 0;
+    int* h_meta = nullptr;        int* hd_meta = nullptr;
+    Candidate* h_cands = nullptr; Candidate* hd_cands = nullptr;
+    int2* h_s23 = nullptr;        int2* hd_s23 = nullptr;  size_t h_cap = 0;
+    // tile skipping: per-frame cell table, network activations, keep flags
+    SnFrame* d_snf = nullptr;     size_t snf_cap = 0;
+    float* d_sn = nullptr;        size_t sn_cap = 0;       // x0, a1, a2, z back to back (1 + 8 + 8 + SLOTS per cell)
+    unsigned char* d_keep = nullptr; size_t keep_cap = 0;  // response tiles, then L1 tiles
+    int* h_kept = nullptr;        int* hd_kept = nullptr;
+    long long tiles = 0, kept = 0;                         // of the last batch (kept = tiles without skipping)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> key;   // what the captured graph was built for
+    int launches = 0;             // kernel launches in the graph
+    int gen = 0;                  // bumped when buffers move (the graph is then recaptured)
+};
+
 class Detector {
 public:
     static constexpr int NSLOTS = 3;
@@ -142,11 +254,58 @@ public:
     ~Detector();
 
     int init(const ccnn_param& p);
+    // One parameter changed through a single-field setter: the detector is rebuilt at the
+    // next Detect with gray_image_only as it stands (CNNDetector's setters only Clear and
+    // change param, cnn_detector_v3.h:404-436; Init and setParam force it on again).
+    void set_fields(const ccnn_param& p)
+    {
+        param = p;
+        built_ = false;
+    }
     int detect(const ccnn_image& img, bool device_ptr, std::vector<ccnn_face>& faces);
     // Detection of n frames with the per-frame launches of consecutive frames pipelined
     // over the slots' streams. faces[i] receives frame i's detections; returns 0, or -1 on a
     // HIP error or an uninitialised detector.
     int detect_batch(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
+    // Batch mode of detect_batch: 1 (default) runs each batch of frames as one graph launch
+    // (ccnn_batch.cpp); 0 pipelines per-frame launches over the slots' streams. Facial
+    // analysis, and a batch whose candidates overflow the batch buffers, take the per-frame
+    // path either way.
+    void set_batch_mode(int m) { batch_mode_ = m; }
+    int batch_mode() const { return batch_mode_; }
+    // Compute precision: 0 the exact FP32 path (the drop-in, bit-exact with the reference);
+    // 1 FP16 stage 1 and BF16 matrix-core stages 2/3; 2 FP16 stage 1 and FP8 stages 2/3.
+    int set_precision(int p)
+    {
+        if (p < 0 || p > 2) return -1;
+        if (p != precision_) batch_.key.clear();
+        precision_ = p;
+        return 0;
+    }
+    int precision() const { return precision_; }
+    // Tile skipping (see ccnn_set_skipnet): params NULL switches it off.
+    int set_skipnet(const float* params, float threshold, int min_face, float scale_factor);
+    void skip_stats(long long* tiles, long long* kept) const
+    {
+        *tiles = batch_.tiles;
+        *kept = batch_.kept;
+    }
+    // Stage-1 tile shape of the plans made from now on (see TileShape); 0 or -1 for a bad id.
+    int set_tile_shape(int id)
+    {
+        if (id < 0 || id >= N_TILE_SHAPES) return -1;
+        shape_ = id;
+        return 0;
+    }
+    int tile_shape_id() const { return shape_; }
+    // Autotuner: runs the batch `reps` times with each tile shape (and each persistent grid
+    // size of the check kernel in {2, 3, 4} workgroups per CU), keeps the fastest and
+    // returns its shape id; ms[s] receives each shape's best batch time (N_TILE_SHAPES
+    // entries, NaN for shapes that failed) and grid_ms[g] each grid's (3 entries).
+    int autotune(int n, const ccnn_image* imgs, bool device_ptr, int reps, float* ms, float* grid_ms);
+    // kernel launches of the last batch graph, graph captures so far
+    int batch_launches() const { return batch_.launches; }
+    int batch_captures() const { return batch_captures_; }
     int clear();
     bool empty() const { return empty_; }
 
@@ -160,6 +319,9 @@ public:
     }
     int last_timing(float* ms) const;
     int last_trace(int* n_levels, int* levels, int cap, float* cand, float* res) const;
+    // stage-1 response maps of the last frame's levels (compute order, each rh x rw,
+    // back to back); returns the float count, -1 if none or cap is too small
+    long last_responses(float* out, size_t cap);
 
     // stage-level access (tests)
     int stage_check(const float* gray, int gw, int gh, int n, const float* scale, const int* x, const int* y,
@@ -171,12 +333,26 @@ private:
     int init_device();
     int build();                                  // Init chain after parameter changes
     int prepare(const ccnn_image& img);           // size guards and re-initialisation of Detect
-    const Plan* plan_for(int gw, int gh);
+    const Plan* plan_for(int gw, int gh, int ncap);
     int ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& plan);
     int enqueue(Slot& s, const ccnn_image& img, bool device_ptr);
+    // launches one planned frame on slot s
+    int enqueue_plan(Slot& s, const ccnn_image& img, bool device_ptr, const Plan* plan, bool pre, int gw, int gh);
+    // the frame's pre-resize, gray size and plan; advances num_scales_
+    const Plan* plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh);
+    int detect_batch_pipelined(int n, const ccnn_image* imgs, bool device_ptr,
+                               std::vector<std::vector<ccnn_face>>& faces);
+    int detect_batch_graph(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
+    // one batch of planned frames through the graph; 1 when the candidates overflowed the
+    // batch buffers (nothing written to faces), 0 on success, -1 on error
+    int run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
+    void release_batch();
     int finish(Slot& s, std::vector<ccnn_face>& faces);
     void release_slot(Slot& s);
     CheckParams check_params() const;
+    std::vector<Det> replay(const LevelDesc* levels, int nl, const Candidate* cands, const CheckResult* res, int count,
+                            bool checks) const;
+    void emit_faces(const std::vector<Det>& rects, bool rescale, std::vector<ccnn_face>& faces) const;
 
     bool empty_ = true;
     bool built_ = false;
@@ -185,7 +361,19 @@ private:
     Settings set_;
     NetWeights net_[4];
     float* d_w_[4] = {nullptr, nullptr, nullptr, nullptr};   // stages 1-3, landmarks
+    float* d_mw_[2] = {nullptr, nullptr};                     // MfmaW tables of stages 2, 3
+    int precision_ = 0;
+    float* d_skip_ = nullptr;      // skip-net weights (FP16-rounded), nullptr when off
+    float skip_threshold_ = 0.5f;
+    int skip_min_face_ = 0;
+    float skip_sf_ = 0.f;
     std::vector<float> scales_;
+    // CNNDetector::num_scales: the scales RunCPUDetect visits. Init sets it to the scale
+    // count, each frame's skip rule lowers it to the first scale whose level is too small,
+    // and Detect restores it only when the frame size differs from max_image_size
+    // (cnn_detector_v3.cpp:603, 2083-2088, 2514-2518): a frame of the maximum size that
+    // follows a smaller one skips the scales the smaller one could not run.
+    int num_scales_ = 0;
     float pre_scale_ = 0.f;
     int ext_w_ = 47, ext_h_ = 51;
     std::vector<Plan*> plans_;
@@ -193,7 +381,11 @@ private:
     int last_slot_ = -1;
     std::vector<float> h_gray_;
 
-    // last frame record
+    // last frame record: its gray image and response maps (device memory of the slot or
+    // batch that ran it), candidates and check results
+    const float* last_gray_ = nullptr;
+    int last_gp_ = 0, last_gw_ = 0, last_gh_ = 0;
+    const float* last_resp_ = nullptr;
     int last_count_ = 0;
     std::vector<Candidate> last_cands_;
     std::vector<CheckResult> last_results_;
@@ -205,6 +397,13 @@ private:
     double acc_ms_[NT] = {};
     int acc_frames_ = 0;
     int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
+    int cus_ = 256;                // compute units of the device
+    int check_per_cu_ = 4;         // persistent k_check workgroups per CU on the batch path
+    int batch_mode_ = 1;
+    long long plan_ids_ = 0;
+    int shape_ = 0;                // stage-1 tile shape of new plans (set_tile_shape, autotune)
+    int batch_captures_ = 0;
+    BatchArena batch_;
     int fa_blocks_ = 64;           // k_facial workgroups (76 KB LDS each); windows dealt round-robin
 };
 

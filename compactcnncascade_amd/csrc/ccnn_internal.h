@@ -46,6 +46,38 @@ using Stage3 = NetShape<6, 4, 12, 24, 7, 8, 17, 2, 8>;
 // 13 outputs: five (x, y) landmarks, gender, smile, glasses
 using Landmarks = NetShape<24, 4, 48, 96, 10, 11, 1, 384, 96, 13>;
 
+// FP16 tap pairs of the stage-1 network for the reduced-precision path, as 32-bit words
+// (two FP16 values each) after the FP32 weights: per L1 map and kernel row (k0, k1) and
+// (k2, k3); per L2 map and row (k0, k1) and (k2, 0); per L3 map and row (k0, k1) and
+// (k2, k3); per hidden unit (w0, w1) and (w2, w3).
+struct Stage1H {
+    static constexpr int oL1 = 0;
+    static constexpr int oL2 = oL1 + Stage1::M1 * Stage1::K1 * 2;
+    static constexpr int oL3 = oL2 + Stage1::M2 * 3 * 2;
+    static constexpr int oHL = oL3 + Stage1::M3 * Stage1::K3H * 2;
+    static constexpr int SIZE = oHL + Stage1::HL * 2;
+};
+
+// Weights of the stage-2/3 networks for the matrix-core path, expanded per lane of the
+// 16x16x32 MFMA operand (lane l holds column l % 16 and K samples 8*(l/16) .. +7):
+// L1: two column tiles (6 maps x 4 pooling-quad offsets, padded to 32) x 64 lanes x 8;
+// L2: three sum pairs x 64 lanes x 8; then the FP8 weight scales of L1 and L2 (powers of
+// two), which the kernels apply before rounding to FP8.
+struct MfmaW {
+    static constexpr int oL1 = 0;
+    static constexpr int oL2 = oL1 + 2 * 64 * 8;
+    static constexpr int oScale = oL2 + 3 * 64 * 8;
+    static constexpr int SIZE = oScale + 2;
+};
+
+// One frame of a skip-net batch (ccnn_skipnet.hip): its grid of 8x8 cells and where its
+// planes start (a layer buffer holds the frames back to back, C planes of h8 x w8 each).
+struct SnFrame {
+    int h8, w8;
+    int off;              // the frame's first cell in the batch (its planes start at off * C)
+    int pos_off;          // the same in the flattened (frame, y, x) numbering of the launches
+};
+
 // One pyramid level as the kernels see it.
 struct LevelDesc {
     int lw, lh;           // level image size
@@ -63,12 +95,49 @@ struct LevelDesc {
     int s1_w, s1_h;       // L1-sum plane size: every pooled L1 position the tiles read
     int l1_tile_off;      // first L1 tile of the level
     int l1_tiles_x, l1_tiles_y;
+    // the level's frame in a batch of frames (gp > 0): frame index, offset (floats) of its
+    // gray image in the batch's gray buffer, pitch and size. gp == 0 on the single-frame
+    // path, where the kernels take the gray image's geometry as arguments.
+    int frame;
+    int gp, gw, gh;
+    long long gray_off;
+};
+
+// A frame of a batch as the batched gray kernel sees it.
+struct FrameDesc {
+    const uint8_t* src;   // 8-bit frame (device memory): gray, BGR or BGRA rows of `step` bytes
+    int w, h, ch, step;
+    // a 1-channel frame that is pre-resized (gray_image_only): the original frame, which
+    // the 8-bit bilinear resize writes into src (w x h, step w)
+    const uint8_t* orig;
+    int ow, oh, ostep;
+    long long gray_off;   // offset (floats) of its FP32 gray image in the batch's gray buffer
+    int gp;               // gray pitch (floats)
+    int qw;               // quads (4 pixels) per row
+    int quads_off;        // first quad of the frame in the batch's quad numbering
+    int aligned;          // 4-byte (16-byte for BGRA) aligned rows: vector loads
 };
 
 // Stage 1 runs in two launches. The first computes the four L1-sum maps (conv, act,
-// pool, neighbour sums) of every level once, S1A_TX x S1A_TY pooled positions per
-// workgroup; the second computes TX x TY response values per workgroup from them.
-constexpr int S1A_TX = 64, S1A_TY = 16;
+// pool, neighbour sums) of every level once, AX x AY pooled positions per workgroup; the
+// second computes TX x TY response values per workgroup from them. The kernels are built
+// for the tile shapes below (shape id = L1 shape index * N_RESP_SHAPES + response shape
+// index); a plan is laid out for one of them, which the autotuner picks per workload.
+// Shape 0 (64 x 16 L1 tiles, 16 x 16 response tiles) is the default.
+constexpr int N_L1_SHAPES = 4, N_RESP_SHAPES = 3, N_TILE_SHAPES = N_L1_SHAPES * N_RESP_SHAPES;
+constexpr int L1_SHAPES[N_L1_SHAPES][2] = {{64, 16}, {32, 16}, {32, 32}, {64, 8}};
+constexpr int RESP_SHAPES[N_RESP_SHAPES][2] = {{16, 16}, {32, 8}, {16, 8}};
+struct TileShape {
+    int ax, ay;           // pooled L1 positions per L1 tile
+    int tx, ty;           // responses per response tile
+};
+inline TileShape tile_shape(int id)
+{
+    if (id < 0 || id >= N_TILE_SHAPES) id = 0;
+    const int a = id / N_RESP_SHAPES, r = id % N_RESP_SHAPES;
+    return TileShape{L1_SHAPES[a][0], L1_SHAPES[a][1], RESP_SHAPES[r][0], RESP_SHAPES[r][1]};
+}
+constexpr int S1A_TX = 64, S1A_TY = 16;      // shape 0
 constexpr int S1A_THREADS = 256;
 constexpr int S1_TX = 16, S1_TY = 16;
 constexpr int S1_THREADS = 256;
@@ -76,6 +145,9 @@ constexpr int S1_THREADS = 256;
 // before ordering it.
 constexpr int CMP_CHUNK = 1024;
 constexpr int CHECK_THREADS = 256;
+// Positions per workgroup of the ordered candidate compaction (256 threads x 16).
+constexpr int CMP_BLOCK = 4096;
+constexpr int CMP_SCAN_THREADS = 1024;
 constexpr int FA_THREADS = 512;
 constexpr int MAX_LEVELS = 512;
 

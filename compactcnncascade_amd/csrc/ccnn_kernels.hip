@@ -66,6 +66,33 @@ __device__ __forceinline__ float lrelu_bn(float conv, const float* __restrict__ 
     return (p[2 * M + m] * fmaxf(0.f, c) + p[3 * M + m]) + p[M + m] * c;
 }
 
+// ---------------------------------------------------------------------------------
+// Reduced-precision stage 1 (FP16 convolutions, FP32 accumulation): the inputs of every
+// convolution and of the hidden layer are rounded to FP16 and multiplied pairwise with
+// FP16 weights by v_dot2_f32_f16, which adds both exact products to an FP32 accumulator;
+// activations, pooling, sums and the tanh stay in FP32 (fused multiply-adds allowed).
+// The FP16 weights are laid out after the FP32 ones (Stage1H, filled by the host).
+// ---------------------------------------------------------------------------------
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ h2_t pk16(float a, float b)
+{
+    h2_t r;
+    r.x = (_Float16)a;
+    r.y = (_Float16)b;
+    return r;
+}
+
+__device__ __forceinline__ h2_t as_h2(uint32_t u) { return __builtin_bit_cast(h2_t, u); }
+
+__device__ __forceinline__ float dot2(h2_t a, h2_t b, float c) { return __builtin_amdgcn_fdot2(a, b, c, false); }
+
+__device__ __forceinline__ float lrelu_bn_fma(float conv, const float* __restrict__ p, int m, int M)
+{
+    const float c = conv + p[m];
+    return __builtin_fmaf(p[M + m], c, __builtin_fmaf(p[2 * M + m], fmaxf(0.f, c), p[3 * M + m]));
+}
+
 // Sample of pyramid level L at (lx, ly), from the FP32 gray image
 // (ImageResizer::preprocessing + Nearest/BilinearInterpolation, scalar tails).
 __device__ __forceinline__ float level_px(const float* __restrict__ g, int gp, int gw, int gh,
@@ -96,6 +123,18 @@ __device__ __forceinline__ float level_px(const float* __restrict__ g, int gp, i
     return (p0 * cx + p1 * fx) * cy + (p2 * cx + p3 * fx) * fy;
 }
 
+// The gray image a level samples: the kernel's arguments on the single-frame path, the
+// level's own frame in a batch (LevelDesc::gp > 0).
+struct GrayRef {
+    const float* g;
+    int gp, gw, gh;
+};
+__device__ __forceinline__ GrayRef level_gray(const float* g, int gp, int gw, int gh, const LevelDesc& L)
+{
+    if (L.gp > 0) return GrayRef{g + L.gray_off, L.gp, L.gw, L.gh};
+    return GrayRef{g, gp, gw, gh};
+}
+
 // ---------------------------------------------------------------------------------
 // Gray conversion
 // ---------------------------------------------------------------------------------
@@ -107,19 +146,13 @@ __device__ __forceinline__ float gray_of(float B, float G, float R)
 
 __device__ __forceinline__ float ub(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
 
-// One thread per 4 pixels of a row (a quad), rows flattened over the grid: 4-byte loads
-// of the frame, one 16-byte store of gray. Needs a 4-byte aligned frame and row step and
-// a gray pitch that is a multiple of 4 floats; the last quad of a row may be partial.
-__global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, int w, int h, int ch, int step,
-                                              float* __restrict__ gray, int gp, int qw)
+// Gray of the quad (4 pixels) at column x of row s of a frame w pixels wide into d: 4-byte
+// loads (16-byte for BGRA) and one 16-byte store when `aligned` and the quad is whole,
+// byte loads otherwise.
+__device__ __forceinline__ void gray_quad(const uint8_t* __restrict__ s, int x, int w, int ch, bool aligned,
+                                          float* __restrict__ d)
 {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= qw * h) return;
-    const int y = q / qw;
-    const int x = (q - y * qw) * 4;
-    const uint8_t* s = img + (size_t)y * step;
-    float* d = gray + (size_t)y * gp + x;
-    if (x + 4 <= w) {
+    if (aligned && x + 4 <= w) {
         float4 v;
         if (ch == 3) {
             const uint32_t* p = reinterpret_cast<const uint32_t*>(s + (size_t)x * 3);
@@ -140,11 +173,43 @@ __global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, i
         }
         *reinterpret_cast<float4*>(d) = v;
     } else {
-        for (int i = 0; x + i < w; ++i) {
+        for (int i = 0; i < 4 && x + i < w; ++i) {
             const uint8_t* t = s + (size_t)(x + i) * ch;
             d[i] = ch == 1 ? (float)t[0] : gray_of((float)t[0], (float)t[1], (float)t[2]);
         }
     }
+}
+
+// One thread per 4 pixels of a row (a quad), rows flattened over the grid: 4-byte loads
+// of the frame, one 16-byte store of gray. Needs a 4-byte aligned frame and row step and
+// a gray pitch that is a multiple of 4 floats; the last quad of a row may be partial.
+__global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, int w, int h, int ch, int step,
+                                              float* __restrict__ gray, int gp, int qw)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= qw * h) return;
+    const int y = q / qw;
+    const int x = (q - y * qw) * 4;
+    gray_quad(img + (size_t)y * step, x, w, ch, true, gray + (size_t)y * gp + x);
+}
+
+// Gray of every frame of a batch: the frames' quads numbered back to back
+// (FrameDesc::quads_off), each thread's frame found by binary search of the table.
+__global__ void __launch_bounds__(256) k_gray_batch(const FrameDesc* __restrict__ frames, int nframes,
+                                                    int total_quads, float* __restrict__ gray)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= total_quads) return;
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (frames[mid].quads_off <= q) lo = mid; else hi = mid - 1;
+    }
+    const FrameDesc& F = frames[lo];
+    const int qq = q - F.quads_off;
+    const int y = qq / F.qw;
+    const int x = (qq - y * F.qw) * 4;
+    gray_quad(F.src + (size_t)y * F.step, x, F.w, F.ch, F.aligned != 0, gray + F.gray_off + (size_t)y * F.gp + x);
 }
 
 // Any alignment: one thread per pixel, byte loads.
@@ -187,6 +252,35 @@ __global__ void __launch_bounds__(256) k_resize_u8(const uint8_t* __restrict__ s
     dst[(size_t)iy * dw + ix] = (uint8_t)(uint32_t)v;
 }
 
+// The pre-resize of frame f of a batch (FrameDesc::orig -> src), as k_resize_u8.
+__global__ void __launch_bounds__(256) k_resize_u8_tab(const FrameDesc* __restrict__ frames, int f)
+{
+    const FrameDesc& F = frames[f];
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    if (ix >= F.w || iy >= F.h) return;
+    const uint32_t sclx = ((uint32_t)F.ow << 12) / (uint32_t)F.w + 1;
+    const uint32_t scly = ((uint32_t)F.oh << 12) / (uint32_t)F.h + 1;
+    const uint32_t y = (uint32_t)iy * scly;
+    uint32_t py = y >> 12;
+    float fy = (float)(y - (py << 12));
+    float cy = 4096.f - fy;
+    fy = fy / 16777216.f;
+    cy = cy / 16777216.f;
+    py = min(py, (uint32_t)(F.oh - 1));
+    if (py + 1 >= (uint32_t)F.oh) py--;
+    const uint32_t x = (uint32_t)ix * sclx;
+    uint32_t px = x >> 12;
+    const float fx = (float)(x - (px << 12));
+    const float cx = 4096.f - fx;
+    px = min(px, (uint32_t)(F.ow - 2));
+    const uint8_t* r0 = F.orig + (size_t)py * F.ostep + px;
+    const uint8_t* r1 = r0 + F.ostep;
+    const float p0 = r0[0], p1 = r0[1], p2 = r1[0], p3 = r1[1];
+    const float v = (p0 * cx + p1 * fx) * cy + (p2 * cx + p3 * fx) * fy;
+    const_cast<uint8_t*>(F.src)[(size_t)iy * F.w + ix] = (uint8_t)(uint32_t)v;
+}
+
 // One level image (tests and getGrayImage-style inspection; the detection path samples
 // levels inside k_stage1 instead of storing them).
 __global__ void __launch_bounds__(256) k_level(const float* __restrict__ g, int gp, int gw, int gh, LevelDesc L,
@@ -210,12 +304,14 @@ __global__ void __launch_bounds__(256) k_level(const float* __restrict__ g, int 
 // ---------------------------------------------------------------------------------
 namespace s1 {
 using N = Stage1;
-constexpr int TX = S1_TX, TY = S1_TY;
-constexpr int P1W = 2 * TX + 8, P1H = 2 * TY + 10;    // L1 sums read by one response tile
-constexpr int P2W = TX + 3, P2H = TY + 4;             // pooled L2 / second sums
-constexpr int P1N = P1W * P1H, P2N = P2W * P2H;
-constexpr int AX = S1A_TX, AY = S1A_TY;               // pooled L1 positions per L1 tile
-constexpr int AW = 2 * AX + 3, AH = 2 * AY + 3;       // level window of one L1 tile
+// a response tile of TX x TY responses reads P1W x P1H L1 sums and computes P2W x P2H
+// pooled L2 positions; an L1 tile of AX x AY pooled positions samples AW x AH level pixels
+template <int TX, int TY>
+struct RespGeo {
+    static constexpr int P1W = 2 * TX + 8, P1H = 2 * TY + 10;
+    static constexpr int P2W = TX + 3, P2H = TY + 4;
+    static constexpr int P1N = P1W * P1H, P2N = P2W * P2H;
+};
 }  // namespace s1
 
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): workgroup b
@@ -244,12 +340,15 @@ __device__ __forceinline__ int find_level(const LevelDesc* __restrict__ L, int n
     return lo;
 }
 
-__global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restrict__ g, int gp, int gw, int gh,
+template <int AX, int AY, bool H>
+__global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restrict__ g0, int gp0, int gw0, int gh0,
                                                            const LevelDesc* __restrict__ levels, int nlevels,
                                                            const float* __restrict__ W, float* __restrict__ s1buf,
-                                                           int* __restrict__ meta, const int2* __restrict__ order)
+                                                           int* __restrict__ meta, const int2* __restrict__ order,
+                                                           const unsigned char* __restrict__ l1_keep)
 {
-    using namespace s1;
+    using N = Stage1;
+    constexpr int AW = 2 * AX + 3, AH = 2 * AY + 3;     // level window of one L1 tile
     __shared__ float s_in[AH * AW];
     __shared__ uint32_t s_row[AH], s_px[AW];
     __shared__ float s_fx[AW], s_cx[AW], s_fy[AH], s_cy[AH];
@@ -270,7 +369,12 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
         li = find_level<BY_L1_TILE>(levels, nlevels, tile);
         t = tile - levels[li].l1_tile_off;
     }
+    // l1_keep (optional): L1 tiles that no kept response tile reads are skipped
+    if (l1_keep && !l1_keep[levels[li].l1_tile_off + t]) return;
     const LevelDesc L = levels[li];
+    const GrayRef gr = level_gray(g0, gp0, gw0, gh0, L);
+    const float* __restrict__ g = gr.g;
+    const int gp = gr.gp, gw = gr.gw, gh = gr.gh;
     const int X0 = (t % L.l1_tiles_x) * AX, Y0 = (t / L.l1_tiles_x) * AY;
     const int x0 = 2 * X0, y0 = 2 * Y0;
 
@@ -317,8 +421,8 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
             s_cy[iy] = cy;
         }
         __syncthreads();
-        constexpr int NS = AW * AH, PER = (NS + S1A_THREADS - 1) / S1A_THREADS, B = 9;
-        static_assert(PER % B == 0, "whole batches");
+        // batches of B samples per thread; past NS the loads are clamped and the stores skipped
+        constexpr int NS = AW * AH, PER = (NS + S1A_THREADS - 1) / S1A_THREADS, B = PER < 9 ? PER : 9;
         if (L.mode == 1) {
 #pragma unroll 1
             for (int b0 = 0; b0 < PER; b0 += B) {
@@ -385,6 +489,32 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
 #pragma unroll
             for (int c = 0; c < 5; ++c) win[r][c] = s_in[(2 * py + r) * AW + 2 * px + c];
         float pm[N::M1];
+        if constexpr (H) {
+            // the window's adjacent pairs (c, c+1) of each row in FP16; tap pairs
+            // (k[r][0], k[r][1]) and (k[r][2], k[r][3]) of each map in Stage1H
+            h2_t hp[5][4];
+#pragma unroll
+            for (int r = 0; r < 5; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) hp[r][c] = pk16(win[r][c], win[r][c + 1]);
+            const uint32_t* kh = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL1;
+#pragma unroll
+            for (int m = 0; m < N::M1; ++m) {
+                float a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dx = q & 1, dy = q >> 1;
+                    float acc = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        acc = dot2(hp[dy + r][dx], as_h2(kh[m * 8 + 2 * r]), acc);
+                        acc = dot2(hp[dy + r][dx + 2], as_h2(kh[m * 8 + 2 * r + 1]), acc);
+                    }
+                    a[q] = lrelu_bn_fma(acc, b1, m, N::M1);
+                }
+                pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+            }
+        } else {
 #pragma unroll
         for (int m = 0; m < N::M1; ++m) {
             const float* k = k1 + m * 16;
@@ -399,6 +529,7 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
             }
             pm[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
         }
+        }
         float* o = out + (size_t)Y * L.s1_w + X;
         o[0] = pm[0] + pm[1];
 #pragma unroll
@@ -407,13 +538,17 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     }
 }
 
+template <int TX, int TY, bool H>
 __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
                                                        const float* __restrict__ W,
                                                        const float* __restrict__ s1buf, float* __restrict__ resp,
                                                        float t1, int* __restrict__ meta, Candidate* __restrict__ cands,
-                                                       Candidate* __restrict__ hcands, int hcap)
+                                                       Candidate* __restrict__ hcands, int hcap,
+                                                       const unsigned char* __restrict__ keep)
 {
-    using namespace s1;
+    using N = Stage1;
+    using G = s1::RespGeo<TX, TY>;
+    constexpr int P1W = G::P1W, P1H = G::P1H, P2W = G::P2W, P1N = G::P1N, P2N = G::P2N;
     static_assert(TX * TY <= S1_THREADS, "one response value per thread");
     __shared__ __align__(16) float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
     __shared__ __align__(16) float s_p2[N::M2 * P2N];     // L2 maps; later the tail weights
@@ -439,6 +574,14 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const int vw = min(TX, L.rw - ox0), vh = min(TY, L.rh - oy0);
     const int qw = vw + 3, qh = vh + 4, nq = qw * qh;
     const float rqw = 1.f / (float)qw, rnq = 1.f / (float)nq, rvw = 1.f / (float)vw;
+    // keep (optional): a skipped tile's responses are -inf, below every threshold
+    if (keep && !keep[tile]) {
+        if (tid < vw * vh) {
+            const int oy = (int)(((float)tid + 0.5f) * rvw), ox = tid - oy * vw;
+            resp[L.resp_off + (oy0 + oy) * L.rw + ox0 + ox] = -INFINITY;
+        }
+        return;
+    }
 
     // the tile's L1 sums: rows 2*oy0 .. +P1H, columns 2*ox0 .. +P1W of each plane
     // (inside the planes: s1_w = 2*TX*tiles_x + 8, s1_h = 2*TY*tiles_y + 10). Every
@@ -447,8 +590,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     {
         static_assert(P1W % 4 == 0 && (2 * TX) % 4 == 0, "float4 rows");
         constexpr int RQ = P1W / 4, NQ = N::M1 * P1H * RQ;
-        constexpr int PER = (NQ + S1_THREADS - 1) / S1_THREADS, B = 7;
-        static_assert(PER <= B, "one batch");
+        constexpr int PER = (NQ + S1_THREADS - 1) / S1_THREADS, B = PER;   // one batch
         const uint32_t plane = (uint32_t)L.s1_w * L.s1_h;
         const float* src = s1buf + L.s1_off + (size_t)(2 * oy0) * L.s1_w + 2 * ox0;
         float4 v[B];
@@ -480,6 +622,35 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
+        if constexpr (H) {
+            // pairs (c, c+1) of each window row, and (c3, 0) for the last tap of the
+            // right-hand quads; per map and row the tap pairs (k0, k1), (k2, 0)
+            h2_t hp[4][3], hz[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hp[r][c] = pk16(win[r][c], win[r][c + 1]);
+                hz[r] = pk16(win[r][3], 0.f);
+            }
+            const uint32_t* kh = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL2;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int m = 2 * i + half;
+                float a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dx = q & 1, dy = q >> 1;
+                    float acc = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        acc = dot2(hp[dy + r][dx], as_h2(kh[m * 6 + 2 * r]), acc);
+                        acc = dot2(dx ? hz[dy + r] : hp[dy + r][2], as_h2(kh[m * 6 + 2 * r + 1]), acc);
+                    }
+                    a[q] = lrelu_bn_fma(acc, b2, m, N::M2);
+                }
+                s_p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+            }
+        } else {
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int m = 2 * i + half;
@@ -494,6 +665,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
                 a[q] = lrelu_bn(acc, b2, m, N::M2);
             }
             s_p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+        }
         }
     }
     __syncthreads();
@@ -514,6 +686,13 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 #pragma unroll
     for (int k = 0; k < WPER; ++k)
         if (tid + k * S1_THREADS < WT) s_wt[tid + k * S1_THREADS] = wt[k];
+    // FP16 path: the L3 and hidden-layer tap pairs follow the FP32 tail weights in LDS
+    uint32_t* s_wh = reinterpret_cast<uint32_t*>(s_wt + WT);
+    if constexpr (H) {
+        static_assert(WT + Stage1H::SIZE - Stage1H::oL3 <= N::M2 * P2N, "FP16 tail weights fit");
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL3;
+        for (int k = tid; k < Stage1H::SIZE - Stage1H::oL3; k += S1_THREADS) s_wh[k] = src[k];
+    }
     __syncthreads();
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
@@ -533,6 +712,28 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         for (int r = 0; r < N::K3H; ++r)
 #pragma unroll
             for (int c = 0; c < N::K3W; ++c) win[r][c] = src[r * P2W + c];
+        if constexpr (H) {
+            static_assert(N::K3W == 4, "two tap pairs per L3 row");
+            h2_t hp[N::K3H][2];
+#pragma unroll
+            for (int r = 0; r < N::K3H; ++r) {
+                hp[r][0] = pk16(win[r][0], win[r][1]);
+                hp[r][1] = pk16(win[r][2], win[r][3]);
+            }
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int m = 2 * i + half;
+                const uint32_t* kh = s_wh + (Stage1H::oL3 - Stage1H::oL3) + m * 2 * N::K3H;
+                float d = 0.f;
+#pragma unroll
+                for (int r = 0; r < N::K3H; ++r) {
+                    d = dot2(hp[r][0], as_h2(kh[2 * r]), d);
+                    d = dot2(hp[r][1], as_h2(kh[2 * r + 1]), d);
+                }
+                a3[m] = lrelu_bn_fma(d, b3, m, N::M3);
+            }
+            continue;
+        }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             const int m = 2 * i + half;
@@ -550,6 +751,23 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     }
     // hidden layer: mulCN_add_tanhW (cnnpp_cplusplus.cpp:646-672, loop cnn_simd_cntk.cpp:874-883)
     float hv[N::HL];
+    if constexpr (H) {
+        static_assert(N::HLN == 4, "two input pairs per hidden unit");
+        h2_t ha[N::M3 - 1];
+#pragma unroll
+        for (int j = 0; j < N::M3 - 1; ++j) ha[j] = pk16(a3[j], a3[j + 1]);
+        const uint32_t* kh = s_wh + (Stage1H::oHL - Stage1H::oL3);
+#pragma unroll
+        for (int gi = 0; gi < N::HLG; ++gi)
+#pragma unroll
+            for (int tt = 0; tt < N::HLS; ++tt) {
+                const int u = N::HLS * gi + tt;
+                float c = dot2(ha[gi], as_h2(kh[2 * u]), Wt[N::oHB + u]);
+                c = dot2(ha[gi + 2], as_h2(kh[2 * u + 1]), c);
+                c = rtanh(c * Wt[N::oHT + gi]);
+                hv[u] = __builtin_fmaf(Wt[N::oHBW + tt], __builtin_fmaf(c, 0.5f, 0.5f), Wt[N::oHBB + tt]);
+            }
+    } else {
 #pragma unroll
     for (int gi = 0; gi < N::HLG; ++gi)
 #pragma unroll
@@ -563,6 +781,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
             c = (c + 1.f) * 0.5f;
             hv[u] = Wt[N::oHBW + tt] * c + Wt[N::oHBB + tt];
         }
+    }
     // output layer: mulC / mulC1_add / tanhW (cnnpp_cplusplus.cpp:952-984, 673-690)
     float o = 0.f;
 #pragma unroll
@@ -571,6 +790,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         for (int tt = 0; tt < N::HLS; ++tt) {
             const float wv = Wt[N::oOW + tt * N::HLG + gi];
             if (gi == 0 && tt == 0) o = hv[0] * wv;
+            else if (H) o = __builtin_fmaf(hv[N::HLS * gi + tt], wv, o);
             else o = hv[N::HLS * gi + tt] * wv + o;
         }
     float c = o + Wt[N::oOB];
@@ -581,8 +801,10 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const int e = (oy0 + oy) * L.rw + ox0 + ox;
     resp[L.resp_off + e] = out;
     // candidate windows (score > t1) go to the frame's list in no particular order; the
-    // host replays them in the reference's order (level, then raster)
-    if (out > t1) {
+    // host replays them in the reference's order (level, then raster). Without meta (the
+    // batch path) the response maps alone are written and the ordered compaction
+    // (k_cmp_*) collects the candidates.
+    if (meta && out > t1) {
         Candidate cd;
         cd.level = li;
         cd.x = ox0 + ox;
@@ -592,6 +814,128 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         cands[idx] = cd;
         // hcands (optional): the same record in mapped host memory, for the first hcap
         if (hcands && idx < hcap) hcands[idx] = cd;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Ordered compaction of the candidate windows (cnn_detector_v3.cpp:1918-1947: a window
+// whose response exceeds t1 becomes a candidate; levels are visited in check order and
+// each map in raster order). On the batch path the response maps of all frames' levels lie
+// back to back in exactly that order, so the candidate list is the stream compaction of
+// the flat response array: k_cmp_count counts the windows per block of CMP_BLOCK
+// positions, k_cmp_scan turns the counts into each block's first output index, and
+// k_cmp_emit writes the windows in position order. Position i = block * CMP_BLOCK +
+// k * 256 + tid (k = 0..15): the loads are coalesced, and inside a block the output order
+// is k, then wave, then lane, which per-wave ballots and their population counts give.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_cmp_count(const float* __restrict__ resp, int n, float t1,
+                                                   int* __restrict__ counts)
+{
+    __shared__ int s_c[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int base = blockIdx.x * CMP_BLOCK;
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = base + k * 256 + tid;
+        v[k] = i < n ? resp[i] : -INFINITY;
+    }
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c += __popcll(__ballot(v[k] > t1));
+    if (lane == 0) s_c[wave] = c;
+    __syncthreads();
+    if (tid == 0) counts[blockIdx.x] = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+// One workgroup: exclusive prefix sums of the nb block counts into offs; meta[0] (and
+// hmeta[0]) = the candidate total, meta[1] = 0 (the check kernel's work queue).
+__global__ void __launch_bounds__(CMP_SCAN_THREADS) k_cmp_scan(const int* __restrict__ counts, int nb,
+                                                               int* __restrict__ offs, int* __restrict__ meta,
+                                                               int* __restrict__ hmeta)
+{
+    constexpr int NW = CMP_SCAN_THREADS / 64;
+    __shared__ int s_w[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (nb + CMP_SCAN_THREADS - 1) / CMP_SCAN_THREADS;
+    const int b0 = min(tid * per, nb), b1 = min(b0 + per, nb);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += counts[b];
+    int v = sum;                                     // inclusive scan inside the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int a = __shfl_up(v, off, 64);
+        if (lane >= off) v += a;
+    }
+    if (lane == 63) s_w[wave] = v;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        before += k < wave ? s_w[k] : 0;
+        total += s_w[k];
+    }
+    int run = before + v - sum;
+    for (int b = b0; b < b1; ++b) {
+        offs[b] = run;
+        run += counts[b];
+    }
+    if (tid == 0) {
+        meta[0] = total;
+        meta[1] = 0;
+        if (hmeta) hmeta[0] = total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cmp_emit(const float* __restrict__ resp, int n, float t1,
+                                                  const int* __restrict__ offs, const LevelDesc* __restrict__ levels,
+                                                  int nlevels, Candidate* __restrict__ cands, int cap,
+                                                  Candidate* __restrict__ hcands, int hcap)
+{
+    __shared__ int s_c[16][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int base = blockIdx.x * CMP_BLOCK;
+    float v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = base + k * 256 + tid;
+        v[k] = i < n ? resp[i] : -INFINITY;
+    }
+    unsigned long long bal[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        bal[k] = __ballot(v[k] > t1);
+        if (lane == 0) s_c[k][wave] = __popcll(bal[k]);
+    }
+    __syncthreads();
+    int run = offs[blockIdx.x];
+    const unsigned long long below = (1ull << lane) - 1;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        int at = run;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) at += w < wave ? s_c[k][w] : 0;
+        if (v[k] > t1) {
+            const int idx = at + __popcll(bal[k] & below);
+            const int i = base + k * 256 + tid;
+            // the level holding position i: the last whose maps start at or before it
+            int lo = 0, hi = nlevels - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (levels[mid].resp_off <= i) lo = mid; else hi = mid - 1;
+            }
+            const LevelDesc& L = levels[lo];
+            const int e = i - L.resp_off;
+            const int y = e / L.rw;
+            Candidate cd;
+            cd.level = lo;
+            cd.x = e - y * L.rw;
+            cd.y = y;
+            cd.score = v[k];
+            if (idx < cap) cands[idx] = cd;
+            if (hcands && idx < hcap) hcands[idx] = cd;
+        }
+        run += s_c[k][0] + s_c[k][1] + s_c[k][2] + s_c[k][3];
     }
 }
 
@@ -635,16 +979,195 @@ static_assert(12 * chk::P2N <= chk::IH * chk::IW && 24 * chk::RN + chk::RN <= ch
                   12 * chk::P2N <= 6 * chk::P1N && chk::HLMAX * chk::RN <= 6 * chk::P1N && 512 <= 6 * chk::P1N,
               "region reuse fits");
 
-template <class N>
-__device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid)
+// ---------------------------------------------------------------------------------
+// Reduced-precision stages 2/3 on the matrix cores. L1 and L2 of each check window are
+// GEMMs on v_mfma_f32_16x16x32_{bf16,fp8_fp8} (FP32 accumulation):
+//   L1: rows = the 528 pooled L1 positions (33 tiles of 16), K = the 25 samples of the
+//       5x5 window under a 2x2 pooling quad (padded to 32), columns = 6 maps x 4 quad
+//       offsets (two tiles of 16); the weights matrix holds each map's 4x4 kernel at its
+//       quad offset inside the 5x5 window, so one product row is the 24 conv outputs
+//       that pool into one position;
+//   L2: rows = the 110 pooled L2 positions (7 tiles), and per pair of L1 sums (a, b)
+//       K = 16 samples of sum a's 4x4 window + 16 of sum b's, columns = sum a's two maps
+//       x 4 quads then sum b's (block-diagonal weights), three products per row tile.
+// The 4x4 quad columns of a map lie in four adjacent lanes of the accumulator, so the
+// 2x2 max-pool is two lane swaps. The weights matrices are expanded per lane by the host
+// (MfmaW) and rounded to the operand type in the kernel. FP8 (e4m3) operands are scaled
+// by powers of two: the weights per layer (host), the L2 input per window (its largest
+// magnitude), and the products are scaled back exactly. L3, the hidden and output layers
+// run in FP32 as on the exact path.
+// ---------------------------------------------------------------------------------
+enum { RP_EXACT = 0, RP_BF16 = 1, RP_FP8 = 2 };
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int RP>
+struct MOp;
+template <>
+struct MOp<RP_BF16> {
+    typedef bf16x8_t T;
+    static __device__ __forceinline__ T pack(const float (&v)[8])
+    {
+        T r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
+        return r;
+    }
+    static __device__ __forceinline__ f32x4_t mma(T a, T b, f32x4_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MOp<RP_FP8> {
+    typedef long T;
+    static __device__ __forceinline__ T pack(const float (&v)[8])
+    {
+        int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+        int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], 0, false);
+        hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+        return (long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+    static __device__ __forceinline__ f32x4_t mma(T a, T b, f32x4_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+    }
+};
+
+// 2^e as a float (|e| < 126)
+__device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
+
+template <class N, int RP>
+__device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const int tid,
+                                const float* __restrict__ mw, int* s_red)
+{
+    using namespace chk;
+    using Op = MOp<RP>;
+    const float* Wl = s.w;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int row = lane & 15, kb = lane >> 4;
+    // this lane's weight operands: L1 (two column tiles), L2 (three sum pairs)
+    float wv[8];
+    typename Op::T b1[2], b2[3];
+    const float sw1 = RP == RP_FP8 ? mw[MfmaW::oScale] : 1.f, sw2 = RP == RP_FP8 ? mw[MfmaW::oScale + 1] : 1.f;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wv[j] = mw[MfmaW::oL1 + (nt * 64 + lane) * 8 + j] * sw1;
+        b1[nt] = Op::pack(wv);
+    }
+#pragma unroll
+    for (int jp = 0; jp < 3; ++jp) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wv[j] = mw[MfmaW::oL2 + (jp * 64 + lane) * 8 + j] * sw2;
+        b2[jp] = Op::pack(wv);
+    }
+    // L1: 33 row tiles of 16 pooled positions over the 4 waves
+    const float inv1 = 1.f / sw1;
+    for (int mt = wave; mt < P1N / 16; mt += CHECK_THREADS / 64) {
+        const int P = mt * 16 + row;
+        const int py = P / P1W, px = P - py * P1W;
+        float a[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = kb * 8 + j;                        // sample t of the 5x5 window
+            a[j] = t < 25 ? in[(2 * py + t / 5) * IW + 2 * px + t % 5] : 0.f;
+        }
+        const typename Op::T av = Op::pack(a);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+            acc = Op::mma(av, b1[nt], acc);
+            const int col = row + 16 * nt, m = col >> 2;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = m < N::M1 ? lrelu_bn_fma(acc[e] * inv1, Wl + N::oB1, m, N::M1) : 0.f;
+                v = fmaxf(v, __shfl_xor(v, 1, 64));
+                v = fmaxf(v, __shfl_xor(v, 2, 64));
+                if ((col & 3) == 0 && m < N::M1) s.s1[m * P1N + mt * 16 + kb * 4 + e] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // neighbour sums in place, and the largest magnitude (the FP8 scale of L2's input)
+    float mx = 0.f;
+    for (int p = tid; p < P1N; p += CHECK_THREADS) {
+        float v[N::M1];
+#pragma unroll
+        for (int m = 0; m < N::M1; ++m) v[m] = s.s1[m * P1N + p];
+        float o[N::M1];
+        o[0] = v[0] + v[1];
+#pragma unroll
+        for (int i = 1; i < N::M1 - 1; ++i) o[i] = v[i - 1] + v[i] + v[i + 1];
+        o[N::M1 - 1] = v[N::M1 - 2] + v[N::M1 - 1];
+#pragma unroll
+        for (int m = 0; m < N::M1; ++m) {
+            s.s1[m * P1N + p] = o[m];
+            mx = fmaxf(mx, fabsf(o[m]));
+        }
+    }
+    float sx = 1.f;
+    if (RP == RP_FP8) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) s_red[wave] = __float_as_int(mx);
+    }
+    __syncthreads();
+    if (RP == RP_FP8) {
+        float m4 = 0.f;
+#pragma unroll
+        for (int w = 0; w < CHECK_THREADS / 64; ++w) m4 = fmaxf(m4, __int_as_float(s_red[w]));
+        // the power of two that brings the largest input into [128, 256)
+        int e = 0;
+        if (m4 > 0.f) e = 8 - (((__float_as_int(m4) >> 23) & 255) - 126);
+        sx = pow2f(min(max(e, -100), 100));
+    }
+    // L2: 7 row tiles of 16 pooled positions (the last one partly), three sum pairs each
+    const float inv2 = 1.f / (sx * sw2);
+    for (int mt = wave; mt < (P2N + 15) / 16; mt += CHECK_THREADS / 64) {
+        const int Q = min(mt * 16 + row, P2N - 1);
+        const int qy = Q / P2W, qx = Q - qy * P2W;
+#pragma unroll
+        for (int jp = 0; jp < 3; ++jp) {
+            // k block kb: samples 8*kb .. 8*kb+7 of sum 2*jp (kb < 2) or 2*jp + 1
+            const float* src = s.s1 + (2 * jp + (kb >> 1)) * P1N;
+            float a[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int t = (kb & 1) * 8 + j;             // sample t of the 4x4 window
+                a[j] = src[(2 * qy + (t >> 2)) * P1W + 2 * qx + (t & 3)] * sx;
+            }
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+            acc = Op::mma(Op::pack(a), b2[jp], acc);
+            const int m = 4 * jp + (row >> 2);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = lrelu_bn_fma(acc[e] * inv2, Wl + N::oB2, m, N::M2);
+                v = fmaxf(v, __shfl_xor(v, 1, 64));
+                v = fmaxf(v, __shfl_xor(v, 2, 64));
+                const int q = mt * 16 + kb * 4 + e;
+                if ((row & 3) == 0 && q < P2N) s.p2()[m * P2N + q] = v;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <class N, int RP = RP_EXACT>
+__device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid,
+                              const float* __restrict__ mw = nullptr, int* s_red = nullptr)
 {
     using namespace chk;
     // the network's weights, staged in LDS (s.w); read the same for every thread in L1,
     // per thread from L2 on
     const float* Wl = s.w;
+    const unsigned ut = (unsigned)tid;
+    if constexpr (RP != RP_EXACT) {
+        check_l1l2_mfma<N, RP>(in, s, tid, mw, s_red);
+    } else {
     // L1 + sums: the first FULL positions one per lane (whole waves), the REM others by
     // one wave with a lane per 2x2-pool quad. Index arithmetic is unsigned throughout.
-    const unsigned ut = (unsigned)tid;
     constexpr unsigned FULL = P1N / 64 * 64, REM = P1N - FULL;
     static_assert(REM == 0 || REM * 4 == 64, "the remainder fills one wave, four lanes a position");
     for (unsigned p = ut; p < FULL; p += CHECK_THREADS) {
@@ -730,6 +1253,7 @@ __device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const in
         }
     }
     __syncthreads();
+    }
     float* s2 = s.s2();
     for (unsigned p = ut; p < P2N; p += CHECK_THREADS) {
         float v[N::M2];
@@ -936,25 +1460,38 @@ __device__ void write_input(ChkSmem& s, uint32_t (&u)[chk::PER], bool remap, boo
 // s3res {-1, -10.f} when stage 3 is skipped. hmeta / hs2 / hs3 (optional): the candidate
 // count and, for the first hcap candidates, the same outcomes in mapped host memory, for
 // decisions taken on the host.
-__global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restrict__ g, int gp, int gw, int gh,
+template <int RP>
+__global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restrict__ g0, int gp0, int gw0, int gh0,
                                                          const LevelDesc* __restrict__ levels,
                                                          const Candidate* __restrict__ cands,
                                                          const int* __restrict__ meta, CheckParams P,
                                                          const float* __restrict__ W2, const float* __restrict__ W3,
                                                          int2* __restrict__ s2res, int2* __restrict__ s3res,
                                                          int* __restrict__ hmeta, int2* __restrict__ hs2,
-                                                         int2* __restrict__ hs3, int hcap)
+                                                         int2* __restrict__ hs3, int hcap, int cap,
+                                                         int* __restrict__ queue, const float* __restrict__ MW2,
+                                                         const float* __restrict__ MW3)
 {
     using namespace chk;
     __shared__ ChkSmem s;
     __shared__ int s_w[CHECK_THREADS / 64];
     __shared__ int s_k1;
+    __shared__ int s_item;
     // work items: (candidate, crop variant) pairs, both variants of a candidate next to
-    // each other so that they are dispatched together
+    // each other so that they are dispatched together. The candidates past cap (the
+    // list's capacity) were not stored: the caller sees meta[0] > cap and redoes the work.
     const int nv = P.double_check ? 2 : 1;
-    const int items = meta[0] * nv;
+    const int items = min(meta[0], cap) * nv;
     if (hmeta && blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = meta[0];
-    for (int item = blockIdx.x; item < items; item += gridDim.x) {
+    // queue (optional): a counter, zero at launch, from which the workgroups take items
+    // in turn (a persistent grid that balances the stage-3 tail); otherwise grid-stride
+    int item = blockIdx.x;
+    if (queue) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        item = s_item;
+    }
+    while (item < items) {
         const int ci = nv == 2 ? item >> 1 : item;
         const int mod = nv == 2 ? item & 1 : 0;
         // the thread index is made opaque per candidate: otherwise every per-thread index
@@ -964,15 +1501,19 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         asm volatile("" : "+v"(tid));
         const size_t slot = (size_t)ci * 2 + mod;
         const Candidate cd = cands[ci];
-        const float scale = levels[cd.level].scale;
-        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gw, gh);
+        const LevelDesc& Lv = levels[cd.level];
+        const float scale = Lv.scale;
+        const GrayRef gr = level_gray(g0, gp0, gw0, gh0, Lv);
+        const float* __restrict__ g = gr.g;
+        const int gp = gr.gp;
+        const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gr.gw, gr.gh);
         // stage-2 weights into LDS; the first barrier of prepare_crop orders them
         stage_weights<Stage2::SIZE>(s.w, W2, tid);
         uint32_t u[PER];
         bool remap;
         prepare_crop(g, gp, cr, P.equalize != 0, s, s_w, u, remap, tid);
         write_input(s, u, remap, false, P.reflection != 0, tid);
-        check_forward<Stage2>(s.in, s, tid);
+        check_forward<Stage2, RP>(s.in, s, tid, MW2, s_w);
         if (tid == 0) {
             float d[RN];
 #pragma unroll
@@ -1001,13 +1542,11 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
                 s3res[slot] = make_int2(-1, __float_as_int(-10.f));
                 if (hs3 && ci < hcap) hs3[slot] = make_int2(-1, __float_as_int(-10.f));
             }
-            __syncthreads();
-            continue;
-        }
+        } else {
         // stage-3 weights; the barrier closing write_input orders them
         stage_weights<Stage3::SIZE>(s.w, W3, tid);
         write_input(s, u, false, true, P.reflection != 0, tid);
-        check_forward<Stage3>(s.in, s, tid);
+        check_forward<Stage3, RP>(s.in, s, tid, MW3, s_w);
         if (tid == 0) {
             float d[RN];
 #pragma unroll
@@ -1025,7 +1564,18 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
             s3res[slot] = make_int2(k2, __float_as_int(m2));
             if (hs3 && ci < hcap) hs3[slot] = make_int2(k2, __float_as_int(m2));
         }
-        __syncthreads();
+        }
+        // the next item: every thread reads it after the barrier, and the previous item's
+        // last LDS reads are done before it is written
+        if (queue) {
+            __syncthreads();
+            if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+            __syncthreads();
+            item = s_item;
+        } else {
+            __syncthreads();
+            item += gridDim.x;
+        }
     }
 }
 
@@ -1324,7 +1874,7 @@ __device__ __noinline__ void landmarks_forward(const float* __restrict__ W, FaSm
 // workgroup b runs the landmarks network on the accepted windows whose number is b modulo
 // the grid size. The facial result goes to results[ci].fa and, for ci < hcap, to the
 // mapped hresults[ci].fa.
-__global__ void __launch_bounds__(FA_THREADS, 4) k_facial(const float* __restrict__ g, int gp, int gw, int gh,
+__global__ void __launch_bounds__(FA_THREADS, 4) k_facial(const float* __restrict__ g0, int gp0, int gw0, int gh0,
                                                           const LevelDesc* __restrict__ levels,
                                                           const Candidate* __restrict__ cands,
                                                           const int* __restrict__ meta, CheckParams P,
@@ -1373,7 +1923,10 @@ __global__ void __launch_bounds__(FA_THREADS, 4) k_facial(const float* __restric
              k += G) {
             const int wi = s_list[k];
             const Candidate cd = cands[wi];
-            const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, s_mod[k], gw, gh);
+            const GrayRef gr = level_gray(g0, gp0, gw0, gh0, levels[cd.level]);
+            const float* __restrict__ g = gr.g;
+            const int gp = gr.gp;
+            const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, levels[cd.level].scale, s_mod[k], gr.gw, gr.gh);
             // nearest-neighbour resize of the crop to 49x51, a copy at that size
             // (image_resize.cpp:328-392, 564-584)
             const bool copy = (cr.rcols == fa::W && cr.rrows == fa::H);
@@ -1419,6 +1972,25 @@ __global__ void __launch_bounds__(FA_THREADS, 4) k_facial(const float* __restric
         }
         base += n;
     }
+}
+
+// The stage-2 or stage-3 network on n 47x51 inputs (tests): its 4x4 response map per
+// input, one workgroup per input.
+template <class N, int RP>
+__global__ void __launch_bounds__(CHECK_THREADS, 4) k_net23(const float* __restrict__ in,
+                                                         const float* __restrict__ W, float* __restrict__ out,
+                                                         const float* __restrict__ MW)
+{
+    using namespace chk;
+    __shared__ ChkSmem s;
+    __shared__ int s_red[CHECK_THREADS / 64];
+    const int tid = threadIdx.x;
+    const float* src = in + (size_t)blockIdx.x * IW * IH;
+    stage_weights<N::SIZE>(s.w, W, tid);
+    for (int i = tid; i < IW * IH; i += CHECK_THREADS) s.in[i] = src[i];
+    __syncthreads();
+    check_forward<N, RP>(s.in, s, tid, MW, s_red);
+    if (tid < RN) out[(size_t)blockIdx.x * RN + tid] = s.out()[tid];
 }
 
 // The landmarks network on one 49x51 input (tests).
@@ -1497,25 +2069,74 @@ hipError_t launch_level(const float* gray, int gp, int gw, int gh, const LevelDe
     return hipGetLastError();
 }
 
+namespace {
+
+template <int AX, int AY, bool H>
+void launch_l1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels, int nl1tiles,
+               const float* W1, float* s1buf, int* meta, const int2* l1_order, hipStream_t st, const hipEvent_t* kev,
+               const unsigned char* l1_keep)
+{
+    if (kev)
+        hipExtLaunchKernelGGL(k_stage1_l1<AX, AY, H>, dim3(nl1tiles), dim3(S1A_THREADS), 0, st, kev[0], kev[1], 0, gray,
+                              gp, gw, gh, d_levels, nlevels, W1, s1buf, meta, l1_order, l1_keep);
+    else
+        k_stage1_l1<AX, AY, H><<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, meta,
+                                                                l1_order, l1_keep);
+}
+
+template <int TX, int TY, bool H>
+void launch_resp(const LevelDesc* d_levels, int nlevels, int ntiles, const float* W1, const float* s1buf, float* resp,
+                 float t1, int* meta, Candidate* cands, Candidate* hcands, int hcap, hipStream_t st,
+                 const hipEvent_t* kev, const unsigned char* keep)
+{
+    if (kev)
+        hipExtLaunchKernelGGL(k_stage1<TX, TY, H>, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels,
+                              nlevels, W1, s1buf, resp, t1, meta, cands, hcands, hcap, keep);
+    else
+        k_stage1<TX, TY, H><<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands, hcands,
+                                                           hcap, keep);
+}
+
+}  // namespace
+
+template <bool H>
+hipError_t launch_stage1_t(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
+                           int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1, int* meta,
+                           Candidate* cands, Candidate* hcands, int hcap, const int2* l1_order, hipStream_t st,
+                           const hipEvent_t* kev, int shape, const unsigned char* keep, const unsigned char* l1_keep)
+{
+    const TileShape t = tile_shape(shape);
+#define L1_ARGS gray, gp, gw, gh, d_levels, nlevels, nl1tiles, W1, s1buf, meta, l1_order, st, kev, l1_keep
+    if (t.ax == 64 && t.ay == 16) launch_l1<64, 16, H>(L1_ARGS);
+    else if (t.ax == 32 && t.ay == 16) launch_l1<32, 16, H>(L1_ARGS);
+    else if (t.ax == 32 && t.ay == 32) launch_l1<32, 32, H>(L1_ARGS);
+    else if (t.ax == 64 && t.ay == 8) launch_l1<64, 8, H>(L1_ARGS);
+    else return hipErrorInvalidValue;
+#undef L1_ARGS
+#define RESP_ARGS d_levels, nlevels, ntiles, W1, s1buf, resp, t1, meta, cands, hcands, hcap, st, kev, keep
+    if (t.tx == 16 && t.ty == 16) launch_resp<16, 16, H>(RESP_ARGS);
+    else if (t.tx == 32 && t.ty == 8) launch_resp<32, 8, H>(RESP_ARGS);
+    else if (t.tx == 16 && t.ty == 8) launch_resp<16, 8, H>(RESP_ARGS);
+    else return hipErrorInvalidValue;
+#undef RESP_ARGS
+    return hipGetLastError();
+}
+
 hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, int nlevels,
                          int nl1tiles, int ntiles, const float* W1, float* s1buf, float* resp, float t1,
                          int* meta, Candidate* cands, Candidate* hcands, int hcap, const int2* l1_order,
-                         hipStream_t st, const hipEvent_t* kev)
+                         hipStream_t st, const hipEvent_t* kev, int shape, bool half, const unsigned char* keep,
+                         const unsigned char* l1_keep)
 {
     if (ntiles <= 0) return hipSuccess;
     // kev (optional): start/stop events of the two launches, stamped when each kernel
-    // starts and ends executing
-    if (kev)
-        hipExtLaunchKernelGGL(k_stage1_l1, dim3(nl1tiles), dim3(S1A_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw,
-                              gh, d_levels, nlevels, W1, s1buf, meta, l1_order);
-    else
-        k_stage1_l1<<<nl1tiles, S1A_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, nlevels, W1, s1buf, meta, l1_order);
-    if (kev)
-        hipExtLaunchKernelGGL(k_stage1, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels, nlevels, W1,
-                              s1buf, resp, t1, meta, cands, hcands, hcap);
-    else
-        k_stage1<<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands, hcands, hcap);
-    return hipGetLastError();
+    // starts and ends executing. shape: the tile shapes the plan was laid out for; half:
+    // FP16 convolutions (W1 then carries the Stage1H pairs after the FP32 weights)
+    if (half)
+        return launch_stage1_t<true>(gray, gp, gw, gh, d_levels, nlevels, nl1tiles, ntiles, W1, s1buf, resp, t1, meta,
+                                     cands, hcands, hcap, l1_order, st, kev, shape, keep, l1_keep);
+    return launch_stage1_t<false>(gray, gp, gw, gh, d_levels, nlevels, nl1tiles, ntiles, W1, s1buf, resp, t1, meta,
+                                  cands, hcands, hcap, l1_order, st, kev, shape, keep, l1_keep);
 }
 
 // Read-back of a frame's candidate list and check results straight into mapped host
@@ -1556,28 +2177,76 @@ hipError_t launch_selftest_tanh(uint32_t lo, uint32_t hi, unsigned long long* ou
     return hipGetLastError();
 }
 
+hipError_t launch_net23(int stage, const float* in, int n, const float* W, float* out, hipStream_t st, int rp,
+                        const float* MW)
+{
+    if (n <= 0) return hipSuccess;
+    if (rp < RP_EXACT || rp > RP_FP8 || (rp != RP_EXACT && !MW)) return hipErrorInvalidValue;
+#define NET23(NN, R) k_net23<NN, R><<<n, CHECK_THREADS, 0, st>>>(in, W, out, MW)
+    if (stage == 1) {
+        if (rp == RP_EXACT) NET23(Stage2, RP_EXACT); else if (rp == RP_BF16) NET23(Stage2, RP_BF16); else NET23(Stage2, RP_FP8);
+    } else {
+        if (rp == RP_EXACT) NET23(Stage3, RP_EXACT); else if (rp == RP_BF16) NET23(Stage3, RP_BF16); else NET23(Stage3, RP_FP8);
+    }
+#undef NET23
+    return hipGetLastError();
+}
+
 hipError_t launch_landmarks(const float* in, const float* W, float* out, hipStream_t st)
 {
     k_landmarks<<<1, FA_THREADS, 0, st>>>(in, W, out);
     return hipGetLastError();
 }
 
+hipError_t launch_resize_u8_tab(const FrameDesc* frames, int f, int dw, int dh, hipStream_t st)
+{
+    dim3 grid((dw + 255) / 256, dh);
+    k_resize_u8_tab<<<grid, 256, 0, st>>>(frames, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_gray_batch(const FrameDesc* frames, int nframes, int total_quads, float* gray, hipStream_t st)
+{
+    if (total_quads <= 0) return hipSuccess;
+    k_gray_batch<<<(total_quads + 255) / 256, 256, 0, st>>>(frames, nframes, total_quads, gray);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const float* resp, int n, float t1, int* counts, int* offs, const LevelDesc* d_levels,
+                          int nlevels, Candidate* cands, int cap, Candidate* hcands, int hcap, int* meta, int* hmeta,
+                          hipStream_t st)
+{
+    const int nb = (n + CMP_BLOCK - 1) / CMP_BLOCK;
+    if (nb > 0) k_cmp_count<<<nb, 256, 0, st>>>(resp, n, t1, counts);
+    k_cmp_scan<<<1, CMP_SCAN_THREADS, 0, st>>>(counts, nb, offs, meta, hmeta);
+    if (nb > 0) k_cmp_emit<<<nb, 256, 0, st>>>(resp, n, t1, offs, d_levels, nlevels, cands, cap, hcands, hcap);
+    return hipGetLastError();
+}
+
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,
-                        CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st, const hipEvent_t* kev)
+                        CheckResult* hresults, int2* hs2, int2* hs3, int hcap, hipStream_t st, const hipEvent_t* kev,
+                        int cap, int* queue, int rp, const float* MW2, const float* MW3)
 {
+    if (rp < RP_EXACT || rp > RP_FP8 || (rp != RP_EXACT && (!MW2 || !MW3))) return hipErrorInvalidValue;
     // kev (optional): start/stop events of k_check, stamped when it starts and ends executing.
     // With hs2/hs3 the outcomes go to mapped host memory and the host takes the decisions
     // (no k_check_decide); otherwise k_check_decide writes results (and, with hmeta, the
     // read-back).
     int* km = hs2 ? hmeta : nullptr;
-    if (kev)
-        hipExtLaunchKernelGGL(k_check, dim3(nblocks), dim3(CHECK_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw, gh,
-                              d_levels, cands, meta, P, W2, W3, s2res, s3res, km, hs2, hs3, hcap);
-    else
-        k_check<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res, km,
-                                                   hs2, hs3, hcap);
+    auto go = [&](auto kern) {
+        if (kev)
+            hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(CHECK_THREADS), 0, st, kev[0], kev[1], 0, gray, gp, gw, gh,
+                                  d_levels, cands, meta, P, W2, W3, s2res, s3res, km, hs2, hs3, hcap, cap, queue, MW2,
+                                  MW3);
+        else
+            kern<<<nblocks, CHECK_THREADS, 0, st>>>(gray, gp, gw, gh, d_levels, cands, meta, P, W2, W3, s2res, s3res, km,
+                                                    hs2, hs3, hcap, cap, queue, MW2, MW3);
+    };
+    if (rp == RP_EXACT) go(k_check<RP_EXACT>);
+    else if (rp == RP_BF16) go(k_check<RP_BF16>);
+    else go(k_check<RP_FP8>);
     if (!hs2)
         k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
                                                               hcands, hresults, hcap);

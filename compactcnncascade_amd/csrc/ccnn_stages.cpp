@@ -110,8 +110,49 @@ int ccnn_stage_landmarks(const float* in, float* out)
     return 0;
 }
 
+int ccnn_stage23(int stage, const float* in, int n, float* out)
+{
+    return ccnn_stage23_ex(stage, in, n, 0, out);
+}
+
+int ccnn_stage23_ex(int stage, const float* in, int n, int precision, float* out)
+{
+    if ((stage != 1 && stage != 2) || n < 0 || (n > 0 && (!in || !out)) || precision < 0 || precision > 2) return -1;
+    if (n == 0) return 0;
+    if (!have_device()) return -1;
+    ccnn::NetWeights net;
+    const unsigned char* b;
+    size_t nb;
+    ccnn::embedded_model(stage, &b, &nb);
+    std::string err;
+    // index_output defaults of FaceDetector::Param (CompactCNNLib.h:160)
+    if (ccnn::load_model(b, nb, stage, 0, net, err) < 0) return -1;
+    std::vector<float> mw;
+    ccnn::mfma_weights(net.w, mw);
+    DevBuf i, w, o, m;
+    const size_t ni = (size_t)n * 47 * 51;
+    if (!i.alloc(sizeof(float) * ni) || !w.alloc(sizeof(float) * net.w.size()) || !o.alloc(sizeof(float) * 16 * n) ||
+        !m.alloc(sizeof(float) * mw.size()))
+        return -1;
+    if (hipMemcpy(i.p, in, sizeof(float) * ni, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(w.p, net.w.data(), sizeof(float) * net.w.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m.p, mw.data(), sizeof(float) * mw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    if (ccnn::launch_net23(stage, i.as<float>(), n, w.as<float>(), o.as<float>(), nullptr, precision,
+                           m.as<float>()) != hipSuccess)
+        return -1;
+    if (hipMemcpy(out, o.p, sizeof(float) * 16 * n, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
 int ccnn_stage1(const float* level, int lw, int lh, float* out)
 {
+    return ccnn_stage1_ex(level, lw, lh, 0, out);
+}
+
+int ccnn_stage1_ex(const float* level, int lw, int lh, int precision, float* out)
+{
+    if (precision < 0 || precision > 2) return -1;
     int rw, rh;
     if (!level || !out || ccnn_stage1_size(lw, lh, &rw, &rh) < 0) return -1;
     if (lw < 23 || lh < 27) return -1;
@@ -139,7 +180,7 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out)
         return -1;
     if (ccnn::launch_stage1(g.as<float>(), lw, lw, lh, l.as<ccnn::LevelDesc>(), 1, L.l1_tiles_x * L.l1_tiles_y,
                             L.tiles_x * L.tiles_y, w.as<float>(), s1.as<float>(), r.as<float>(), INFINITY, nullptr,
-                            nullptr, nullptr, 0, nullptr, nullptr) != hipSuccess)
+                            nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, precision > 0) != hipSuccess)
         return -1;
     if (hipMemcpy(out, r.p, sizeof(float) * (size_t)rw * rh, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     return 0;

@@ -1,0 +1,229 @@
+"""Learned tile skipping (SkipNet): training data from the plain cascade, training on the
+GPU with the library's own forward/backward kernels, and the model file.
+
+The network (compactcnncascade_amd/csrc/ccnn_skipnet.hip) reads the gray frame
+downsampled by 8 (each 8x8 block's mean / 255 - 0.5) and gives, per 8x8 frame cell and
+per pyramid scale index s < SLOTS, the logit that the level of scale s has a window that
+the cascade accepts whose centre lies in that cell. The labels come from the plain FP32
+cascade itself: every candidate window the stage-2/3 checks accept (the stage-1 response
+map thresholded at t1, then CPUCheckDetect; cnn_detector_v3.cpp:1485-1818, 1918-1947)
+marks its centre cell and the 8 cells around it.
+
+At detection time the batch path runs the network on every frame, keeps each stage-1
+response tile whose largest probability over its windows' cells reaches the threshold,
+and runs the plain cascade on those tiles only; the levels without a slot, and every
+tile the network does not rule out with that confidence, run as usual.
+
+    python -m compactcnncascade_amd.skipnet train --frames 20000 --out models/skipnet_min20_sf115.npz
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+if os.path.dirname(PKG) not in sys.path:
+    sys.path.insert(0, os.path.dirname(PKG))
+
+from compactcnncascade_amd import synth  # noqa: E402
+
+SLOTS = 24
+NPAR = 8 * 9 + 8 + 8 * 8 * 9 + 8 + SLOTS * 8 * 9 + SLOTS
+DEFAULT_MODEL = os.path.join(PKG, "models", "skipnet_min20_sf115.npz")
+TRAIN_WORKLOAD = dict(w=1280, h=720, lam=10.0, min_size=20, max_size=300)
+MIN_FACE, SCALE_FACTOR = 20, 1.15
+
+
+def frame_gray(frame):
+    """FP32 gray of a BGR/BGRA/gray frame (image_proc.cpp:99-102, left to right)."""
+    f = np.asarray(frame)
+    if f.ndim == 2:
+        return f.astype(np.float32)
+    b, g, r = (f[:, :, k].astype(np.float32) for k in range(3))
+    return (np.float32(0.114) * b + np.float32(0.587) * g) + np.float32(0.299) * r
+
+
+def down8(gray):
+    """The network input: the mean of each 8x8 block (partial blocks at the right and
+    bottom edges over their own pixels) / 255 - 0.5, as k_sn_down8 computes it."""
+    h, w = gray.shape
+    h8, w8 = (h + 7) // 8, (w + 7) // 8
+    pad = np.zeros((h8 * 8, w8 * 8), np.float64)
+    cnt = np.zeros_like(pad)
+    pad[:h, :w] = gray
+    cnt[:h, :w] = 1
+    s = pad.reshape(h8, 8, w8, 8).sum(axis=(1, 3))
+    n = cnt.reshape(h8, 8, w8, 8).sum(axis=(1, 3))
+    return (s / n / 255.0 - 0.5).astype(np.float32)
+
+
+def scales_for(min_face, max_h, scale_factor, pattern_h=27):
+    """InitScales' float32 recurrence (cnn_detector_v3.cpp:289-346) for max_face 0."""
+    f32 = np.float32
+    mx = min(f32(3.0), f32(pattern_h) / f32(min_face))
+    mn = min(mx, f32(pattern_h) / f32(max_h))
+    sf = f32(scale_factor)
+    out, s = [], mx
+    while s > mn:
+        out.append(float(s))
+        s = f32(s / sf)
+    return out
+
+
+def labels_from_trace(shape, trace, scales):
+    """uint8 labels [SLOTS, h8, w8] from one frame's trace (levels, candidates, check
+    results): accepted windows' centre cells and their 8 neighbours."""
+    h, w = shape
+    h8, w8 = (h + 7) // 8, (w + 7) // 8
+    y = np.zeros((SLOTS, h8, w8), np.uint8)
+    lv = trace["levels"]
+    for c, r in zip(trace["cand"], trace["res"]):
+        if r[0] < 0.5:
+            continue
+        scl = int(lv[int(c[0]), 0])
+        if scl >= SLOTS:
+            continue
+        s = scales[scl]
+        u = int(np.floor((4.0 * c[1] + 11.5) / s / 8.0))
+        v = int(np.floor((4.0 * c[2] + 13.5) / s / 8.0))
+        y[scl, max(0, v - 1):min(h8, v + 2), max(0, u - 1):min(w8, u + 2)] = 1
+    return y
+
+
+def _gen(seed):
+    cfg = TRAIN_WORKLOAD
+    return synth.synth_frame(seed, cfg["w"], cfg["h"], cfg["lam"], cfg["min_size"], cfg["max_size"])[0]
+
+
+def make_dataset(api, n, seed0=500000, workers=16, log=print):
+    """n synthetic 1280x720 frames labelled by the plain GPU cascade: (x0 [n, 90, 160],
+    labels [n, SLOTS, 90, 160], faces found)."""
+    cfg = TRAIN_WORKLOAD
+    det = api.FaceDetector()
+    if det.Init(api.Param(max_image_size=(cfg["w"], cfg["h"]), min_face_height=MIN_FACE,
+                          scale_factor=SCALE_FACTOR)) != 0:
+        raise RuntimeError("FaceDetector.Init failed")
+    scales = scales_for(MIN_FACE, cfg["h"], SCALE_FACTOR)
+    h8, w8 = (cfg["h"] + 7) // 8, (cfg["w"] + 7) // 8
+    xs = np.zeros((n, h8, w8), np.float32)
+    ys = np.zeros((n, SLOTS, h8, w8), np.uint8)
+    faces = 0
+    t0 = time.time()
+    chunk = 256
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        for c0 in range(0, n, chunk):
+            frames = list(ex.map(_gen, range(seed0 + c0, seed0 + min(n, c0 + chunk))))
+            for k, f in enumerate(frames):
+                cnt, _ = det.Detect(f)
+                faces += max(cnt, 0)
+                tr = det.last_trace()
+                xs[c0 + k] = down8(frame_gray(f))
+                ys[c0 + k] = labels_from_trace(f.shape[:2], tr, scales)
+            log("dataset %d/%d frames, %.0f s" % (min(n, c0 + chunk), n, time.time() - t0))
+    return xs, ys, faces
+
+
+def init_params(seed=1):
+    """He-style initial weights, zero biases, output biases at -2 (mostly-skip prior)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, np.sqrt(2.0 / 9), 8 * 9), np.zeros(8), rng.normal(0, np.sqrt(2.0 / 72), 8 * 8 * 9),
+             np.zeros(8), rng.normal(0, np.sqrt(2.0 / 72), SLOTS * 8 * 9), np.full(SLOTS, -2.0)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _lib(api):
+    L = api.load()
+    P = ctypes.c_void_p
+    L.ccnn_skipnet_train.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_uint, P]
+    L.ccnn_skipnet_grad.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P]
+    L.ccnn_skipnet_forward.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def train(api, xs, ys, epochs=8, batch=64, lr=3e-3, pos_weight=20.0, seed=7, params=None):
+    """Trains on the GPU (Adam); returns (params, loss per epoch)."""
+    L = _lib(api)
+    p = np.ascontiguousarray(init_params(seed) if params is None else params, np.float32).copy()
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.uint8)
+    n, h8, w8 = xs.shape
+    losses = np.zeros(epochs, np.float32)
+    if L.ccnn_skipnet_train(_p(p), _p(xs), _p(ys), n, h8, w8, epochs, batch, lr, pos_weight, seed, _p(losses)) != 0:
+        raise RuntimeError("ccnn_skipnet_train failed")
+    return p, losses
+
+
+def grad(api, params, xs, ys, pos_weight):
+    """(loss, gradient) of one forward/backward pass on the GPU."""
+    L = _lib(api)
+    p = np.ascontiguousarray(params, np.float32)
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.uint8)
+    g = np.zeros(NPAR, np.float32)
+    loss = np.zeros(1, np.float32)
+    n, h8, w8 = xs.shape
+    if L.ccnn_skipnet_grad(_p(p), _p(xs), _p(ys), n, h8, w8, pos_weight, _p(g), _p(loss)) != 0:
+        raise RuntimeError("ccnn_skipnet_grad failed")
+    return float(loss[0]), g
+
+
+def forward(api, params, xs):
+    """Logits [n, SLOTS, h8, w8] on the GPU."""
+    L = _lib(api)
+    p = np.ascontiguousarray(params, np.float32)
+    xs = np.ascontiguousarray(xs, np.float32)
+    n, h8, w8 = xs.shape
+    z = np.zeros((n, SLOTS, h8, w8), np.float32)
+    if L.ccnn_skipnet_forward(_p(p), _p(xs), n, h8, w8, _p(z)) != 0:
+        raise RuntimeError("ccnn_skipnet_forward failed")
+    return z
+
+
+def to_fp16(params):
+    """The inference weights: the trained FP32 weights rounded to FP16."""
+    return np.asarray(params, np.float32).astype(np.float16).astype(np.float32)
+
+
+def save(path, params, meta):
+    np.savez(path, params=np.asarray(params, np.float32), meta=json.dumps(meta))
+
+
+def load(path=DEFAULT_MODEL):
+    """(params float32 [NPAR], meta dict) of a model file."""
+    with np.load(path, allow_pickle=False) as z:
+        return np.array(z["params"], np.float32), json.loads(str(z["meta"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("--frames", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--out", default=DEFAULT_MODEL)
+    args = ap.parse_args()
+    from compactcnncascade_amd import api
+    t0 = time.time()
+    xs, ys, faces = make_dataset(api, args.frames)
+    print(json.dumps({"dataset_frames": args.frames, "faces": faces, "positive_fraction": float(ys.mean()),
+                      "seconds": round(time.time() - t0, 1)}), flush=True)
+    t1 = time.time()
+    p, losses = train(api, xs, ys, epochs=args.epochs)
+    meta = {"min_face": MIN_FACE, "scale_factor": SCALE_FACTOR, "slots": SLOTS, "frames": args.frames,
+            "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "loss": [float(v) for v in losses],
+            "train_seconds": round(time.time() - t1, 1)}
+    save(args.out, p, meta)
+    print(json.dumps(meta), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -95,6 +95,80 @@ int ccnn_detect_device(ccnn_detector* d, ccnn_face* faces, int capacity, const c
 int ccnn_detect_batch(ccnn_detector* d, int n, const ccnn_image* imgs, int device, ccnn_face* faces,
                       int capacity, int* counts);
 
+/* How ccnn_detect_batch runs a batch (no counterpart in the reference). mode 1 (the
+ * default): every pyramid level of every frame goes through ONE captured HIP graph launch
+ * (gray of all frames, stage 1 over the (frame, level, tile) work list, ordered compaction
+ * of the candidate windows, stages 2/3 on a persistent grid), the graph captured once per
+ * batch layout (frame sizes, channels) and replayed; mode 0: per-frame launches pipelined
+ * over three streams. Both give each frame the detections of one ccnn_detect call in
+ * turn. Facial analysis, and batches with more candidate windows than the batch buffers
+ * hold, run per frame in either mode. 0, or -1 for a bad mode. */
+int ccnn_set_batch_mode(ccnn_detector* d, int mode);
+/* Compute precision of the detector (no counterpart in the reference):
+ *   CCNN_PRECISION_FP32       the exact FP32 path (default): results bit-identical to the
+ *                             reference's CPU path;
+ *   CCNN_PRECISION_FP16_BF16  stage 1 with FP16 convolution inputs and weights (products
+ *                             summed in FP32, v_dot2_f32_f16); the stage-2/3 L1 and L2
+ *                             layers as BF16 GEMMs on the matrix cores with FP32
+ *                             accumulation;
+ *   CCNN_PRECISION_FP16_FP8   the same with FP8 (e4m3) operands for stages 2/3, scaled
+ *                             per layer by powers of two.
+ * Reduced precisions change scores and may change the detection set; recall against the
+ * FP32 path is stated by the project's tests and benchmark. 0, or -1 for a bad value. */
+enum { CCNN_PRECISION_FP32 = 0, CCNN_PRECISION_FP16_BF16 = 1, CCNN_PRECISION_FP16_FP8 = 2 };
+int ccnn_set_compute_precision(ccnn_detector* d, int precision);
+int ccnn_get_compute_precision(const ccnn_detector* d);
+
+/* Stage-1 tile shapes (no counterpart in the reference). The two stage-1 kernels are
+ * built for CCNN_TILE_SHAPES combinations of an L1 tile (pooled L1 positions per
+ * workgroup: 64x16, 32x16, 32x32, 64x8) and a response tile (responses per workgroup:
+ * 16x16, 32x8, 16x8); shape id = L1 index * 3 + response index, 0 the default. The
+ * results do not depend on the shape. set: for the frames planned from now on; 0, or -1
+ * for a bad id. get: the current id, the shape's sizes in ax/ay/tx/ty (may be NULL). */
+#define CCNN_TILE_SHAPES 12
+int ccnn_set_tile_shape(ccnn_detector* d, int shape);
+int ccnn_get_tile_shape(const ccnn_detector* d, int* ax, int* ay, int* tx, int* ty);
+/* Autotuner of the batch path: runs ccnn_detect_batch over the n frames (host images, or
+ * device pointers when device != 0) reps + 1 times per stage-1 tile shape, keeps the
+ * fastest shape, then does the same over the persistent grid of the stage-2/3 kernel (2,
+ * 3 or 4 workgroups per CU) and keeps the fastest. shape_ms[CCNN_TILE_SHAPES] and
+ * grid_ms[3] (may be NULL) receive each one's best batch time in ms (NaN if it failed).
+ * Returns the chosen shape id, or -1. */
+int ccnn_autotune(ccnn_detector* d, int n, const ccnn_image* imgs, int device, int reps, float* shape_ms,
+                  float* grid_ms);
+/* Learned tile skipping (no counterpart in the reference). A proposal network (SkipNet:
+ * three 3x3 convolutions, 1 -> 8 -> 8 -> CCNN_SKIP_SLOTS channels, ReLU, on the gray
+ * frame downsampled by 8) gives per 8x8 frame cell and per pyramid scale index s <
+ * CCNN_SKIP_SLOTS the logit that level s has a window the cascade accepts there. With a
+ * network set, the batch path (ccnn_set_batch_mode 1) runs it on every frame and computes
+ * only the stage-1 tiles whose largest probability over their windows reaches
+ * `threshold`; the other tiles' windows are dropped. Levels with scale index >=
+ * CCNN_SKIP_SLOTS, and detectors whose min_face_height / scale_factor differ from the
+ * ones the network was trained for, run plain. params: CCNN_SKIP_PARAMS floats (w1
+ * [8][1][3][3], b1 [8], w2 [8][8][3][3], b2 [8], w3 [SLOTS][8][3][3], b3 [SLOTS]), used
+ * rounded to FP16; NULL switches skipping off. 0, or -1. */
+#define CCNN_SKIP_SLOTS 24
+#define CCNN_SKIP_PARAMS 2416
+int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int min_face, float scale_factor);
+/* Stage-1 response tiles in the last batch and how many of them ran. 0, or -1. */
+int ccnn_skip_stats(const ccnn_detector* d, long long* tiles, long long* kept);
+/* Training and checking of the proposal network on the GPU with the library's own
+ * kernels (forward, backward, weighted binary cross-entropy, Adam). x0: n inputs of h8 x
+ * w8 floats; labels: n x CCNN_SKIP_SLOTS x h8 x w8 bytes (0/1); loss = mean over all
+ * outputs of pos_weight * softplus(-z) for label 1, softplus(z) for label 0.
+ * grad: one forward/backward pass, gradient (CCNN_SKIP_PARAMS floats) and loss;
+ * forward: the logits (n x SLOTS x h8 x w8); train: `epochs` passes of Adam (lr) over
+ * shuffled batches of `batch` frames, params updated in place, epoch_loss[epochs] (may be
+ * NULL) the mean batch loss of each pass. All return 0, or -1. */
+int ccnn_skipnet_grad(const float* params, const float* x0, const unsigned char* labels, int n, int h8, int w8,
+                      float pos_weight, float* grad, float* loss);
+int ccnn_skipnet_forward(const float* params, const float* x0, int n, int h8, int w8, float* logits);
+int ccnn_skipnet_train(float* params, const float* x0, const unsigned char* labels, int n, int h8, int w8,
+                       int epochs, int batch, float lr, float pos_weight, unsigned seed, float* epoch_loss);
+
+/* Kernel launches in the last captured batch graph and the number of captures so far. */
+int ccnn_batch_stats(const ccnn_detector* d, int* launches, int* captures);
+
 /* FaceDetector::Clear (:355-372): 0, or -1 if empty. */
 int ccnn_clear(ccnn_detector* d);
 /* FaceDetector::isEmpty (:374-377) */
@@ -137,6 +211,12 @@ int ccnn_stage_level(const float* gray, int gw, int gh, int lw, int lh, int mode
  * out: rh x rw with (rw, rh) from ccnn_stage1_size. */
 int ccnn_stage1_size(int lw, int lh, int* rw, int* rh);
 int ccnn_stage1(const float* level, int lw, int lh, float* out);
+/* As ccnn_stage1 in a compute precision of this library (no counterpart in the
+ * reference): 0 the exact FP32 path (ccnn_stage1), 1 or 2 the reduced-precision stage 1
+ * (FP16 convolution and hidden-layer inputs and weights, products summed in FP32 by
+ * v_dot2_f32_f16; activations, pooling and tanh in FP32). */
+int ccnn_stage1_ex(const float* level, int lw, int lh, int precision, float* out);
+
 /* Stage-2/3 check of candidate windows on a gray image (CPUCheckDetect without the drop
  * logic): per candidate i, scale[i] and response-map position (x[i], y[i]); res[8*i..]
  * = {flag, mod, k1, k2, m1, m2, n2, n3}: acceptance, crop variant of the final evaluation,
@@ -144,6 +224,17 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out);
  * network evaluations performed. Settings from p. */
 int ccnn_stage_check(const ccnn_param* p, const float* gray, int gw, int gh, int n,
                      const float* scale, const int* x, const int* y, float* res);
+
+/* The stage-2 (stage 1) or stage-3 (stage 2) network (ConvNeuralNetwork::Forward on a
+ * 47x51 check input, cnn_simd_cntk.cpp:964, as CPUCheckDetect runs it,
+ * cnn_detector_v3.cpp:1646-1700) on n inputs of 47x51 floats (row-major, 47 columns)
+ * laid out back to back; out: the 4x4 response map of each (16 floats per input). */
+int ccnn_stage23(int stage, const float* in, int n, float* out);
+/* As ccnn_stage23 in a compute precision of this library (no counterpart in the
+ * reference): 0 the exact FP32 path, 1 L1 and L2 on the matrix cores with BF16 operands,
+ * 2 with FP8 (e4m3) operands scaled per layer by powers of two; FP32 accumulation, L3 and
+ * the hidden and output layers in FP32. */
+int ccnn_stage23_ex(int stage, const float* in, int n, int precision, float* out);
 
 /* The facial-analysis network (cnn4landmarks_cntk.bin, run by CPUCheckDetect on accepted
  * windows when Param::facial_analysis is set, cnn_detector_v3.cpp:1731-1756) on one 49x51
@@ -158,6 +249,12 @@ int ccnn_stage_landmarks(const float* in, float* out);
  * NULL to query the counts. */
 int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
                     float* cand, float* res);
+
+/* Stage-1 response maps of the frame that ccnn_last_trace describes: the levels in its
+ * order, each a dense rh x rw map (ConvNeuralNetwork::Forward's response map of that
+ * level, cnn_detector_v3.cpp:2109-2112), back to back. Returns the float count (out may
+ * be NULL to query it), or -1 when there is no such frame or cap is smaller. */
+long ccnn_last_responses(ccnn_detector* d, float* out, size_t cap);
 
 /* Facial analysis of the last ccnn_detect* call's candidates (same order as
  * ccnn_last_trace): out[4*i..] = {ok, gender, smile, glasses}; ok 0 marks a window the

@@ -58,6 +58,10 @@ class OcTrace(ctypes.Structure):
                 ("n_raw", _I)]
 
 
+class OcState(ctypes.Structure):
+    _fields_ = [("max_w", _I), ("max_h", _I), ("num_scales", _I), ("gray_only", _I)]
+
+
 def default_params(w, h, **kw):
     """FaceDetector::Param defaults (prebuild/include/CompactCNNLib.h:98-165) with the
     frame size as max_image_size."""
@@ -92,6 +96,10 @@ class Oracle:
         L.oc_sort_desc.argtypes = [_P, _P, _I]
         L.oc_detect.argtypes = [ctypes.POINTER(OcModel)] * 4 + [ctypes.POINTER(OcParams), _P, _I, _I, _I, _I,
                                                                 _P, _I, ctypes.POINTER(OcTrace)]
+        L.oc_detect_state.argtypes = [ctypes.POINTER(OcModel)] * 4 + [ctypes.POINTER(OcParams),
+                                                                      ctypes.POINTER(OcState), _P, _I, _I, _I, _I,
+                                                                      _P, _I, ctypes.POINTER(OcTrace)]
+        L.oc_state_init.argtypes = [ctypes.POINTER(OcState), ctypes.POINTER(OcParams)]
         L.oc_check_candidate.argtypes = [ctypes.POINTER(OcModel)] * 2 + [ctypes.POINTER(OcParams), _P, _I, _I, _I,
                                                                          _F, _I, _I, _P]
         self.models = []
@@ -215,6 +223,49 @@ class Oracle:
         return faces, trace
 
 
+    def new_state(self, params):
+        """Detector state after FaceDetector::Init with params (an OcState)."""
+        st = OcState()
+        self.lib.oc_state_init(ctypes.byref(st), ctypes.byref(_oc_params(params)))
+        return st
+
+    def detect_state(self, img, params, state):
+        """Detection of one frame on a detector in `state` (updated in place); faces
+        [n,8] as detect."""
+        p = _oc_params(params)
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape[:2]
+        ch = 1 if img.ndim == 2 else img.shape[2]
+        cap = 100000
+        raw = np.zeros(cap * 8, np.int32)
+        n = self.lib.oc_detect_state(*(ctypes.byref(m) for m in self.models), ctypes.byref(p), ctypes.byref(state),
+                                     _ptr(img), w, h, ch, w * ch, _ptr(raw), cap, None)
+        if n < 0 or n > cap:
+            raise RuntimeError("oracle detect_state failed (%d)" % n)
+        rec = raw[:8 * n].reshape(n, 8)
+        faces = np.zeros((n, 8), np.float32)
+        faces[:, :4] = rec[:, :4]
+        faces[:, 4] = rec[:, 4].view(np.float32)
+        faces[:, 5:] = rec[:, 5:]
+        return faces
+
+    def detect_seq(self, frames, params, min_face_sets=None):
+        """Detection of each frame in turn on ONE detector (its max_image_size, num_scales
+        and gray_image_only carry from call to call, as CNNDetector's do). min_face_sets[i]
+        > 0: setMinObjectHeight before frame i (the detector is Cleared and Inits at the
+        next Detect with gray_image_only as it stands). Returns a list of faces arrays
+        [n,8] as detect."""
+        params = dict(params)
+        st = self.new_state(params)
+        out = []
+        for i, img in enumerate(frames):
+            if min_face_sets is not None and min_face_sets[i] > 0:
+                params["min_face"] = int(min_face_sets[i])
+                st.num_scales = -1
+            out.append(self.detect_state(img, params, st))
+        return out
+
+
 class Ref:
     """The reference CPU path compiled from its own sources (oracle/Makefile)."""
 
@@ -234,6 +285,7 @@ class Ref:
         L.ref_detect_ex.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _I]
         L.ref_detect_loop.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _I]
         L.ref_detect_loop.restype = ctypes.c_long
+        L.ref_detect_seq.argtypes = [_P, _P, _P, _P, _I, _P, _I, _P]
         self.cnn = [L.ref_cnn_create(os.path.join(MODEL_DIR, MODEL_FILES[i]).encode(), INDEX_OUTPUT[i])
                     for i in range(4)]
         if not all(self.cnn):
@@ -314,3 +366,24 @@ class Ref:
         ch = 1 if frames.ndim == 3 else frames.shape[3]
         ip, fp = self._pack(params)
         return self.lib.ref_detect_loop(_ptr(ip), _ptr(fp), _ptr(frames), nf, w, h, ch, w * ch, reps)
+
+    def detect_seq(self, frames, params, min_face_sets=None, cap=20000):
+        """Detection of each frame in turn on ONE reference detector (state carried
+        across calls; min_face_sets as Oracle.detect_seq). Returns a list of faces arrays
+        [n,8] as detect."""
+        frames = [np.ascontiguousarray(f, np.uint8) for f in frames]
+        sets = min_face_sets if min_face_sets is not None else [0] * len(frames)
+        meta = np.array([[f.shape[1], f.shape[0], 1 if f.ndim == 2 else f.shape[2],
+                          f.shape[1] * (1 if f.ndim == 2 else f.shape[2]), int(m)] for f, m in zip(frames, sets)],
+                        np.int32)
+        buf = np.ascontiguousarray(np.concatenate([f.reshape(-1) for f in frames]))
+        ip, fp = self._pack(params)
+        out = np.zeros((len(frames), cap, 8), np.float32)
+        counts = np.zeros(len(frames), np.int32)
+        rc = self.lib.ref_detect_seq(_ptr(ip), _ptr(fp), _ptr(buf), _ptr(meta), len(frames), _ptr(out), cap,
+                                     _ptr(counts))
+        if rc < 0:
+            raise RuntimeError("reference detect_seq failed (%d)" % rc)
+        if (counts > cap).any():
+            raise RuntimeError("reference detect_seq: more faces than cap")
+        return [out[i, :counts[i]].copy() for i in range(len(frames))]

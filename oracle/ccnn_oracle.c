@@ -1056,9 +1056,26 @@ static int fd_median(const det_t* d, int key)
     return r;
 }
 
+void oc_state_init(oc_state* s, const oc_params* p)
+{
+    s->max_w = p->max_w;
+    s->max_h = p->max_h;
+    s->num_scales = -1;
+    s->gray_only = 1;
+}
+
 int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
               const oc_params* p, const unsigned char* img, int w, int h, int ch, int step,
               oc_face* out, int cap, oc_trace* trace)
+{
+    oc_state s;
+    oc_state_init(&s, p);
+    return oc_detect_state(m1, m2, m3, mfa, p, &s, img, w, h, ch, step, out, cap, trace);
+}
+
+int oc_detect_state(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
+                    const oc_params* p, oc_state* st, const unsigned char* img, int w, int h, int ch, int step,
+                    oc_face* out, int cap, oc_trace* trace)
 {
     dctx_t c;
     memset(&c, 0, sizeof(c));
@@ -1070,15 +1087,31 @@ int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const 
     const int ext_w = c.e.detect_mode != 0 ? m2->min_w + 11 : m1->min_w;
     const int ext_h = c.e.detect_mode != 0 ? m2->min_h + 11 : m1->min_h;
     if (w < ext_w || h < ext_h) return 0;                          /* :2494-2497 */
+    if (w > st->max_w || h > st->max_h) {                          /* :2499-2513: Clear, Init */
+        st->max_w = w;
+        st->max_h = h;
+        if (st->gray_only && ch != 1) st->gray_only = 0;
+        st->num_scales = -1;
+    } else if (w != st->max_w || h != st->max_h) {
+        st->num_scales = -1;                                        /* :2514-2518 */
+    }
     oc_params pp = *p;
-    if (w > pp.max_w || h > pp.max_h) { pp.max_w = w; pp.max_h = h; }   /* :2499-2513 */
-    /* gray_image_only is forced on by the API (CompactCNNLibAPI_v2.cpp:131); a colour
-     * image turns it off and re-initialises (cnn_detector_v3.cpp:2531-2547). */
+    pp.max_w = st->max_w;
+    pp.max_h = st->max_h;
     float scales[512], pre = 0.f;
-    int n_scales = init_scales(&pp, m1->min_h, 1, scales, 512, &pre);
+    int n_scales = init_scales(&pp, m1->min_h, st->gray_only, scales, 512, &pre);
     if (n_scales < 0) return -1;
-    if (pre > 0.f && ch != 1) n_scales = init_scales(&pp, m1->min_h, 0, scales, 512, &pre);
-    if (n_scales < 0) return -1;
+    if (pre > 0.f && ch != 1) {
+        /* a colour image turns gray_image_only off, Clears and detects again on a fresh
+         * Init (cnn_detector_v3.cpp:2540-2546) */
+        st->gray_only = 0;
+        st->num_scales = -1;
+        n_scales = init_scales(&pp, m1->min_h, 0, scales, 512, &pre);
+        if (n_scales < 0) return -1;
+    }
+    /* the scales RunCPUDetect visits this time, and the count it leaves for the next call */
+    const int ncap = st->num_scales < 0 ? n_scales : OC_MIN(st->num_scales, n_scales);
+    int next_scales = ncap;
 
     const unsigned char* src = img;
     int sw = w, sh = h, sstep = step;
@@ -1099,9 +1132,13 @@ int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const 
     int rw[512], rh[512], ran[512];
     for (int scl = n_scales - 1; scl >= 0; --scl) {
         resp[scl] = NULL; ran[scl] = 0;
+        if (scl >= ncap) continue;
         int lw, lh;
         size_mul(sw, sh, scales[scl], &lw, &lh);
-        if (lw < ext_w || lh < ext_h) continue;                     /* :2083-2088 */
+        if (lw < ext_w || lh < ext_h) {                             /* :2083-2088 */
+            next_scales = OC_MIN(scl, next_scales);
+            continue;
+        }
         float* lvl = (float*)malloc(sizeof(float) * (size_t)lw * lh);
         int mode = scales[scl] != 1.f ? (scales[scl] > 0.7f ? 0 : 1) : 2;   /* :2093-2104 */
         if (mode == 2) oc_resize_f32(gray, sw, sh, sw, lvl, lw, lh, lw, 0);
@@ -1117,6 +1154,7 @@ int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const 
         }
         free(lvl);
     }
+    st->num_scales = next_scales;
     /* RunCheckDetectAsync (:2012-2061): levels scl_max -> 0 */
     for (int scl = n_scales - 1; scl >= 0; --scl)
         if (ran[scl]) run_check_detect(&c, scl, scales[scl], resp[scl], rw[scl], rh[scl]);

@@ -107,11 +107,31 @@ typedef struct {
     int n_raw;                   /* detections before Merger */
 } oc_trace;
 
-/* Full detection of one frame with the reference's semantics.
- * Returns the face count (<= cap written), 0 for images below 47x51, -1 on error. */
+/* Full detection of one frame with the reference's semantics, on a freshly initialised
+ * detector. Returns the face count (<= cap written), 0 for images below 47x51, -1 on error. */
 int oc_detect(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
               const oc_params* p, const unsigned char* img, int w, int h, int ch, int step,
               oc_face* out, int cap, oc_trace* trace);
+
+/* Detector state that CNNDetector carries from one Detect call to the next:
+ *   max_w, max_h  param.max_image_size, grown by a larger frame (cnn_detector_v3.cpp:2499-2513);
+ *   num_scales    the scale count RunCPUDetect visits: Init sets it to the scale count
+ *                 (:603), the skip rule lowers it to the first scale whose level is too
+ *                 small (:2083-2088), and Detect resets it only when the frame's size
+ *                 differs from max_image_size (:2514-2518). -1 stands for "all scales";
+ *   gray_only     AdvancedParam::gray_image_only: forced on by the API's Init and setParam
+ *                 (CompactCNNLibAPI_v2.cpp:131), turned off for good by a colour frame
+ *                 that is larger than max_image_size (:2504-2507) or that meets the
+ *                 pre-resize (:2540-2546). */
+typedef struct { int max_w, max_h; int num_scales; int gray_only; } oc_state;
+
+/* The state after FaceDetector::Init with p. */
+void oc_state_init(oc_state* s, const oc_params* p);
+
+/* As oc_detect, on a detector in state s (updated). */
+int oc_detect_state(const oc_model* m1, const oc_model* m2, const oc_model* m3, const oc_model* mfa,
+                    const oc_params* p, oc_state* s, const unsigned char* img, int w, int h, int ch, int step,
+                    oc_face* out, int cap, oc_trace* trace);
 
 /* One candidate's check (CPUCheckDetect without the drop logic): results of mod 0 and,
  * when mod 0 fails and double_check is on, mod 1. res[0..9] =

@@ -203,11 +203,13 @@ int ref_detect(const int* iparams, const float* fparams,
     return n;
 }
 
-// As ref_detect with facial analysis selectable (iparams[10]); out: 8 values per face
-// (x, y, w, h, score, gender, smile, glasses). gender / smile / glasses are reported the
-// way FaceDetector::Detect does (CompactCNNLibAPI_v2.cpp:296-319): the facial_data
-// entries of the detection sorted by the key in descending order, the element at
-// size/2; -1 when there are none.
+// 8 values per face (x, y, w, h, score, gender, smile, glasses), at most cap written.
+// gender / smile / glasses are reported the way FaceDetector::Detect does
+// (CompactCNNLibAPI_v2.cpp:296-319): the facial_data entries of the detection sorted by
+// the key in descending order, the element at size/2; -1 when there are none.
+static int write_faces(std::vector<CNNDetector::Detection>& dets, float* out, int cap);
+
+// As ref_detect with facial analysis selectable (iparams[10]); out as write_faces.
 int ref_detect_ex(const int* iparams, const float* fparams,
                   const unsigned char* img, int w, int h, int ch, int step,
                   float* out, int cap)
@@ -219,6 +221,40 @@ int ref_detect_ex(const int* iparams, const float* fparams,
     int rc = d->Detect(dets, image);
     delete d;
     if (rc < 0) return rc;
+    return write_faces(dets, out, cap);
+}
+
+// Detect over a sequence of frames on ONE detector, whose state (max_image_size,
+// num_scales, gray_image_only) carries from call to call as in the API's video loop
+// (prebuild/examples/main.cpp:36-111). Frames lie back to back in `frames`;
+// meta[5*i..] = {w, h, channels, step, min_face}: a min_face > 0 is set with
+// setMinObjectSize before the frame (CompactCNNLibAPI_v2.cpp:426-431). Frame i's faces go
+// to out + 8*cap*i (as write_faces), its face count to counts[i]. The detector is built
+// for the first frame's channel count (colour first: gray_image_only off, see make_detector).
+int ref_detect_seq(const int* iparams, const float* fparams, const unsigned char* frames, const int* meta, int n,
+                   float* out, int cap, int* counts)
+{
+    if (n <= 0) return 0;
+    CNNDetector* d = make_detector(iparams, fparams, meta[2], iparams[10] != 0);
+    if (!d) return -1;
+    size_t off = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        const int w = meta[5 * i], h = meta[5 * i + 1], ch = meta[5 * i + 2], step = meta[5 * i + 3];
+        if (meta[5 * i + 4] > 0) d->setMinObjectSize(Size(meta[5 * i + 4], meta[5 * i + 4]));
+        std::vector<CNNDetector::Detection> dets;
+        SIMD::Image_8u image(w, h, ch, const_cast<unsigned char*>(frames + off), step);
+        const int rc = d->Detect(dets, image);
+        if (rc < 0) { delete d; return rc; }
+        counts[i] = write_faces(dets, out + (size_t)8 * cap * i, cap);
+        off += (size_t)step * h;
+    }
+    delete d;
+    return 0;
+}
+
+static int write_faces(std::vector<CNNDetector::Detection>& dets, float* out, int cap)
+{
     int n = (int)dets.size();
     int m = n < cap ? n : cap;
     for (int i = 0; i < m; ++i)

@@ -95,8 +95,12 @@ def params_for(frame, overrides):
 
 def api_param(frame, overrides):
     """The same settings as a compactcnncascade_amd.api.Param."""
+    return api_param_of(params_for(frame, overrides))
+
+
+def api_param_of(p):
+    """compactcnncascade_amd.api.Param of an oracle parameter dict."""
     from compactcnncascade_amd.api import Param
-    p = params_for(frame, overrides)
     return Param(max_image_size=(p["max_w"], p["max_h"]), min_face_height=p["min_face"],
                  max_face_height=p["max_face"], min_neighbors=p["min_neighbors"],
                  detect_precision=p["precision"], drop_detect=bool(p["drop_detect"]),
@@ -108,3 +112,56 @@ def api_param(frame, overrides):
 def frame_digest(frame):
     import hashlib
     return hashlib.sha256(np.ascontiguousarray(frame).tobytes()).hexdigest()[:16]
+
+
+# Sequences of frames detected in turn on ONE detector, whose state carries from call to
+# call: max_image_size grows with a larger frame, and num_scales (the scales RunCPUDetect
+# visits) is lowered by each frame's skip rule and restored only when the frame size
+# differs from max_image_size (cnn_detector_v3.cpp:603, 2083-2088, 2499-2518). A frame
+# of the maximum size after a smaller one therefore skips the scales the smaller one
+# could not run. name -> (frame builders, parameter overrides, max_image_size[,
+# min_face set with setMinObjectHeight before each frame, 0 for none]).
+def _big_faces(seed):
+    return synth.synth_frame(seed, 640, 480, 4, 150, 260)[0]
+
+
+def _small(seed):
+    return synth.synth_frame(seed + 100, 400, 300, 2, 40, 120)[0]
+
+
+SEQ_CASES = {
+    "seq_max_small_max_s45": ([lambda: _big_faces(45), lambda: _small(45), lambda: _big_faces(45),
+                               lambda: _big_faces(49)], dict(min_face=40), (640, 480)),
+    "seq_max_small_max_s41_min20": ([lambda: _big_faces(41), lambda: _small(41), lambda: _big_faces(41)],
+                                    dict(min_face=20), (640, 480)),
+    # the first frame is smaller than max_image_size, the second larger (max grows)
+    "seq_grow_s41": ([lambda: _small(41), lambda: _big_faces(41), lambda: _small(41), lambda: _big_faces(41)],
+                     dict(min_face=40), (400, 300)),
+    # a colour frame turns gray_image_only off; setMinObjectHeight keeps it off, so the
+    # following 1-channel frame is not pre-resized (cnn_detector_v3.h:417-436)
+    "seq_setter_gray_after_colour": ([lambda: vga(11, smin=20), lambda: _gray(vga(11, smin=20))],
+                                     dict(min_face=40), (640, 480), [0, 36]),
+}
+
+# The benchmark workloads as frame sequences on one detector (the order DetectBatch and
+# bench.py use): name -> ([(workload, count)], parameter overrides, max_image_size).
+WORKLOAD_SEQS = {
+    # Full HD drop-in: 64 frames 1920x1080, faces 40..240
+    "fhd_min40_max240_x64": ([("fhd_min40_max240", 64)], dict(min_face=40, max_face=240), (1920, 1080)),
+    # the mixed-size batch: 8 each of 640x480, 1280x720, 1920x1080, 3840x2160, faces >= 20,
+    # one detector initialised for the largest frame
+    "mixed_batch_x32": ([("mixed_vga", 8), ("mixed_hd", 8), ("mixed_fhd", 8), ("mixed_uhd", 8)],
+                        dict(min_face=20), (3840, 2160)),
+}
+
+
+def seq_params(overrides, max_size):
+    from oracle.bind import default_params
+    return default_params(max_size[0], max_size[1], **overrides)
+
+
+def workload_frames(spec):
+    out = []
+    for name, n in spec:
+        out.extend(synth.workload_frame(name, i)[0] for i in range(n))
+    return out

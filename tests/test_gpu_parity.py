@@ -90,6 +90,57 @@ def test_stage1_bitexact(api, oracle):
         assert np.array_equal(bits(a), bits(b))
 
 
+RESPONSE_CASES = {
+    "vga_s0_min40": CASES["vga_s0_min40"],
+    "fhd_w0_min40_max240": CASES["fhd_w0_min40_max240"],
+    "photo3_min20": CASES["photo3_min20"],
+    # min face 27: the first scale is exactly 1, a copy level
+    "photo3_min27": (CASES["photo3_min20"][0], dict(min_face=27)),
+}
+
+
+@pytest.mark.parametrize("name", list(RESPONSE_CASES))
+def test_detect_response_maps_bitexact(api, oracle, name):
+    # the response map of every level as a real Detect computes it: the fused sampler of
+    # k_stage1_l1 (copy, nearest-neighbour and bilinear levels) feeding both stage-1
+    # kernels, against the oracle's resize + forward of the same level
+    build, over = RESPONSE_CASES[name]
+    f = build()
+    d = api.FaceDetector()
+    assert d.Init(api_param(f, over)) == 0
+    d.Detect(f)
+    levels = d.last_trace()["levels"]
+    maps = d.last_responses()
+    g = oracle.gray(f)
+    modes = set()
+    for (scl, lw, lh, rw, rh, mode), m in zip(levels, maps):
+        print(name, "level", scl, lw, lh, mode, flush=True)
+        lvl = g if mode == 2 else oracle.resize(g, int(lw), int(lh), int(mode))
+        exp = oracle.forward(0, lvl)
+        assert m.shape == exp.shape == (rh, rw)
+        assert np.array_equal(bits(m), bits(exp)), (name, scl, mode)
+        modes.add(int(mode))
+    assert len(levels) > 5
+    if name == "photo3_min20":
+        assert modes == {0, 1}
+    if name == "photo3_min27":
+        assert modes == {0, 1, 2}
+
+
+def test_stage23_maps_bitexact(api, oracle, vectors):
+    # the full 4x4 response maps of the stage-2 and stage-3 networks: the reference's
+    # vectors and random inputs (u8-valued, as the check feeds them) against the oracle
+    rng = np.random.default_rng(77)
+    x = np.concatenate([vectors["s2_in"][None], np.round(rng.random((15, 51, 47)) * 255).astype(np.float32)])
+    for stage in (1, 2):
+        got = api.stage23(stage, x)
+        for i in range(len(x)):
+            exp = oracle.forward(stage, x[i])
+            assert np.array_equal(bits(got[i]), bits(exp)), (stage, i)
+    assert np.array_equal(bits(api.stage23(1, vectors["s2_in"][None])[0]), bits(vectors["s2_out"]))
+    assert np.array_equal(bits(api.stage23(2, vectors["s2_in"][None])[0]), bits(vectors["s3_out"]))
+
+
 def test_stage1_shift_invariance(api):
     # moving the level by 4 pixels moves the response map by one cell
     rng = np.random.default_rng(9)
@@ -347,15 +398,14 @@ def test_batch_matches_single_frame_results(api, golden):
             assert cnt == len(exp)
             assert [[int(r["x"]), int(r["y"]), int(r["width"]), int(r["height"])] for r in faces] == \
                 [e[:4] for e in exp]
-    # frames of several sizes through one detector in one batch
+    # frames of several sizes through one detector in one batch: the same as Detect calls
+    # in turn on one detector (whose max_image_size and num_scales carry from frame to frame)
     d2 = api.FaceDetector()
     assert d2.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
     batch = d2.DetectBatch(images=frames)
-    singles = []
-    for f in frames:
-        d3 = api.FaceDetector()
-        assert d3.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
-        singles.append(d3.Detect(f))
+    d3 = api.FaceDetector()
+    assert d3.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+    singles = [d3.Detect(f) for f in frames]
     for (cb, fb), (cs, fs) in zip(batch, singles):
         assert cb == cs and np.array_equal(fb, fs)
 

@@ -154,82 +154,13 @@ def test_resize_bilinear_closed_form(oracle):
         assert np.array_equal(bits(oracle.resize(src, dw, dh, 1)), bits(exp.astype(f32)))
 
 
-def _net64(m, x):
-    """Textbook float64 restatement of one cascade network on image x (the model's wiring
-    from the binary format, no rounding-order detail)."""
-    import ctypes
-
-    def arr(p, n):
-        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_float)), shape=(n,)).astype(np.float64)
-
-    M = list(m.maps)
-    kw, kh = list(m.kw), list(m.kh)
-    K = [arr(m.kern[l], M[l] * kw[l] * kh[l]).reshape(M[l], kh[l], kw[l]) for l in range(3)]
-    B = [arr(m.conv_b[l], M[l]) for l in range(3)]
-    W1 = [arr(m.lrelu_w1[l], M[l]) for l in range(3)]
-    W2 = [arr(m.lrelu_w2[l], M[l]) for l in range(3)]
-    BB = [arr(m.bn_b[l], M[l]) for l in range(3)]
-
-    def conv(img, k):
-        H, W = img.shape
-        r, c = k.shape
-        out = np.zeros((H - r + 1, W - c + 1))
-        for i in range(r):
-            for j in range(c):
-                out += k[i, j] * img[i:i + H - r + 1, j:j + W - c + 1]
-        return out
-
-    def act(v, l, i):
-        v = v + B[l][i]
-        return W1[l][i] * v + W2[l][i] * np.maximum(v, 0) + BB[l][i]
-
-    def pool(v):
-        h, w = v.shape[0] // 2 * 2, v.shape[1] // 2 * 2
-        v = v[:h, :w]
-        return np.maximum(np.maximum(v[0::2, 0::2], v[0::2, 1::2]), np.maximum(v[1::2, 0::2], v[1::2, 1::2]))
-
-    def sums(maps):
-        n = len(maps)
-        return [maps[max(0, i - 1):min(n, i + 2)] for i in range(n)]
-
-    x = x.astype(np.float64)
-    p1 = [pool(act(conv(x, K[0][i]), 0, i)) for i in range(M[0])]
-    s1 = [sum(g) for g in sums(p1)]
-    p2 = [pool(act(conv(s1[i // 2], K[1][i]), 1, i)) for i in range(M[1])]
-    s2 = [sum(g) for g in sums(p2)]
-    a3 = [act(conv(s2[i // 2], K[2][i]), 2, i) for i in range(M[2])]
-    S, G, N = m.hl_scale, m.hl // m.hl_scale, m.connect
-    # weights per hidden unit: `connect` of them, or one per L3 map when fully connected
-    hw = arr(m.hl_w, m.hl * m.hl_nin).reshape(m.hl, m.hl_nin)[:, :N]
-    hb, ht = arr(m.hl_b, m.hl), arr(m.hl_tanh_w, G)
-    bw, bb = arr(m.hl_bn_w, S), arr(m.hl_bn_b, S)
-    outs = range(m.ol) if m.index_output < 0 else [m.index_output]
-
-    def rt(c):
-        return np.sign(c) * (1 - 1 / (1 + np.abs(c) + c * c + 1.41645 * c ** 4))
-
-    h = []
-    for g in range(G):
-        for t in range(S):
-            u = S * g + t
-            c = hb[u] + sum(a3[g + j] * hw[u, j] for j in range(N))
-            h.append(bw[t] * (0.5 * rt(c * ht[g]) + 0.5) + bb[t])
-    res = []
-    for k in outs:
-        ow = arr(m.ol_w, m.ol * m.hl).reshape(m.ol, m.hl)[k]
-        ob = arr(m.ol_b, m.ol)[k]
-        o = sum(h[S * g + t] * ow[t * G + g] for g in range(G) for t in range(S))
-        v = rt((o + ob) * m.ol_tanh_w)
-        res.append(0.5 * v + 0.5 if m.af_scale == 0 else m.af_scale * v)
-    return res[0] if m.index_output >= 0 else np.stack(res)
-
-
 @pytest.mark.parametrize("stage,shape", [(0, (51, 47)), (0, (90, 75)), (1, (51, 47)), (2, (51, 47)), (3, (51, 49))])
 def test_networks_match_float64_restatement(oracle, stage, shape):
     rng = np.random.default_rng(10 + stage)
     x = (rng.random(shape) * 255).round().astype(np.float32)
     got = oracle.forward(stage, x).astype(np.float64)
-    exp = _net64(oracle.models[stage], x)
+    from oracle.net64 import forward64
+    exp = forward64(oracle.models[stage], x)
     assert got.shape == exp.shape
     assert np.max(np.abs(got - exp)) < 2e-5 * 1.7159
     if stage == 3:
@@ -332,3 +263,26 @@ def test_model_loader_rejects_bad_files(oracle):
     bad = bytearray(b)
     bad[0:4] = np.float32(2.0).tobytes()                                                # version 2.0
     assert oracle.lib.oc_model_load(ctypes.byref(OcModel()), bytes(bad), len(bad), 1) == -1
+
+
+# ---------------------------------------------------------------------------------
+# detector state across calls (max_image_size, num_scales, gray_image_only)
+# ---------------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["seq_max_small_max_s45", "seq_max_small_max_s41_min20", "seq_grow_s41",
+                                  "seq_setter_gray_after_colour"])
+def test_oracle_sequences_match_reference_goldens(oracle, name):
+    from cases import SEQ_CASES, seq_params
+    with open(os.path.join(GOLDEN, "sequences.json")) as f:
+        g = json.load(f)["cases"][name]
+    builders, over, mx = SEQ_CASES[name][:3]
+    sets = SEQ_CASES[name][3] if len(SEQ_CASES[name]) > 3 else None
+    frames = [b() for b in builders]
+    assert [frame_digest(f) for f in frames] == g["digests"]
+    out = oracle.detect_seq(frames, seq_params(over, mx), min_face_sets=sets)
+    for i, faces in enumerate(out):
+        assert faces[:, :4].astype(int).tolist() == [e[:4] for e in g["faces"][i]], (name, i)
+        assert [int(v) for v in bits(faces[:, 4])] == g["score_bits"][i], (name, i)
+    # the carried state matters here: a fresh detector per frame gives other boxes
+    if name != "seq_setter_gray_after_colour":
+        fresh = [oracle.detect(f, seq_params(over, mx))[0] for f in frames]
+        assert any(len(a) != len(b) or not np.array_equal(a, b) for a, b in zip(out, fresh))
