@@ -17,7 +17,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libccnn.so")
 MODELS = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin", "cnn4landmarks_cntk.bin"]
 
-SOURCES = ["ccnn_kernels.hip", "ccnn_skipnet.hip", "ccnn_detector.cpp", "ccnn_batch.cpp",
+SOURCES = ["ccnn_kernels.hip", "ccnn_noalloc.hip", "ccnn_skipnet.hip", "ccnn_detector.cpp", "ccnn_batch.cpp",
            "ccnn_api.cpp", "ccnn_stages.cpp", "ccnn_convert.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CCNN_OFFLOAD_ARCH", "gfx950")

@@ -217,6 +217,19 @@ int ccnn_stage1(const float* level, int lw, int lh, float* out);
  * v_dot2_f32_f16; activations, pooling and tanh in FP32). */
 int ccnn_stage1_ex(const float* level, int lw, int lh, int precision, float* out);
 
+/* ccnn_stage1 on device buffers, allocating nothing. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). level and out are device pointers
+ * (lh x lw floats in, rh x rw out, both dense); the stage-1 weights, the level
+ * descriptor and the L1-sum planes live in a fixed device array of
+ * CCNN_NOALLOC_SCRATCH_FLOATS floats (64 MiB) that each GPU holds until the process
+ * ends, which limits the level to about 16 million pixels (5,376 x 3,024 at 16:9): larger levels are
+ * rejected with -1 and a message. Runs on the null stream and returns after the
+ * result is written. Calls on one GPU share its array, so calls that overlap on one GPU
+ * (several host threads) race. 0, or -1. */
+#define CCNN_NOALLOC_SCRATCH_FLOATS (16u << 20)
+int ccnn_stage1_noalloc(const float* level, int lw, int lh, float* out);
+
 /* Stage-2/3 check of candidate windows on a gray image (CPUCheckDetect without the drop
  * logic): per candidate i, scale[i] and response-map position (x[i], y[i]); res[8*i..]
  * = {flag, mod, k1, k2, m1, m2, n2, n3}: acceptance, crop variant of the final evaluation,
