@@ -1,21 +1,30 @@
 """Benchmark of the MI355X compact-CNN-cascade detection path.
 
-Default workload (one GPU): 64 seeded 1920x1080 BGR frames, scale factor 1.15, faces
-40..240 px, default FaceDetector parameters, frames resident in GPU memory before the
-timed region. A step is one DetectBatch pass over the 64 frames (per-frame launches,
-consecutive frames pipelined over three streams); `value` is frames/s over K steps.
-Beside it the line reports one synchronous Detect at a time, per-kernel rooflines from a
-single-frame pass, the 4K / mixed-size / 640x480 / deep-pyramid / facial-analysis legs,
-the headline frames handed over in host memory (PCIe upload included) and the
-reference's CPU path on a bounded sample. Prints one JSON line.
+Headline (`value`): 64 seeded 1920x1080 BGR frames, scale factor 1.15, faces 40..240 px,
+default FaceDetector parameters, frames resident in GPU memory before the timed region.
+A step is one DetectBatch call over the 64 frames on the exact FP32 path (bit-identical
+to the reference CPU path): the batch graph (one captured HIP graph launch per batch:
+gray, stage 1 over every (frame, level, tile), ordered candidate compaction, stages 2/3
+on a persistent grid) or per-frame launches pipelined over three streams, whichever is
+faster on the box (both are reported). `value` = frames/s over K steps.
+
+Beside it, the line reports: one synchronous Detect at a time; per-kernel rooflines from
+a single-frame pass; the configurations of the benchmark set -- the 4K leg, the 32-frame
+mixed-size batch (FP32, FP16/BF16, FP16/FP8, launches per batch, recall against FP32),
+the 4800x2400 deep pyramid (FP32/BF16/FP8 with recall), learned tile skipping on held-out
+1920x1080 and 3840x2160 frames with faces >= 20 (plain, skip, skip + FP16/BF16, with
+recall, tiles run and the gain over the plain FP32 cascade), 640x480, facial analysis,
+host frames over PCIe -- and the reference's CPU path on the host's cores (one core, and
+one process per core). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--frames F] [--gpus 1] [--facial]
-                    [--no-secondary] [--no-cpu-baseline]
+                    [--no-secondary] [--no-cpu-baseline] [--profile-pass N]
 """
 import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 from concurrent.futures import ProcessPoolExecutor
@@ -29,14 +38,28 @@ from compactcnncascade_amd import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+FP8_PEAK_TFLOPS = 5000.0
 # The exact path evaluates every multiply and add as its own rounded VALU op (no FMA,
 # no packed FP32): 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T ops/s.
 VALU_EXACT_PEAK_TOPS = 39.3
+STAGE2_MAC = 278496
+STAGE3_MAC = 276672
+
+
+def progress(msg):
+    """One line on stderr per finished part of the run (the JSON line goes to stdout)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
 
 
 def _frame(args):
     name, idx = args
     return synth.workload_frame(name, idx)[0]
+
+
+def _frame_seed(args):
+    seed, w, h = args
+    return synth.synth_frame(seed, w, h, 10.0, 20, 300)[0]
 
 
 def make_frames(name, n):
@@ -45,37 +68,114 @@ def make_frames(name, n):
         return np.stack(list(ex.map(_frame, [(name, i) for i in range(n)])))
 
 
-def secondary_fps(api, torch, names, per_name, steps, facial=False):
-    """Frames/s of DetectBatch over frames of the named workloads (all frames resident in
-    GPU memory, one detector initialised for the largest frame), best of `steps` passes;
-    facial: with facial analysis."""
-    frames = []
-    for name in names:
-        frames.extend(make_frames(name, per_name))
-    dev = [torch.from_numpy(f).to("cuda:0") for f in frames]
+def held_out_frames(w, h, n, seed0):
+    """Seeded frames outside every training set (seeds from 900000; the skip network was
+    trained on seeds from 500000)."""
+    workers = min(16, max(1, (os.cpu_count() or 1)))
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_frame_seed, [(seed0 + i, w, h) for i in range(n)]))
+
+
+def to_device(torch, frames):
+    dev = [torch.from_numpy(np.ascontiguousarray(f)).to("cuda:0") for f in frames]
     torch.cuda.synchronize()
-    cfg = synth.WORKLOADS[names[-1]]
-    W = max(f.shape[1] for f in frames)
-    H = max(f.shape[0] for f in frames)
-    det = api.FaceDetector()
-    if det.Init(api.Param(max_image_size=(W, H), min_face_height=cfg["min_face"], max_face_height=cfg["max_face"],
-                          scale_factor=cfg["scale_factor"], facial_analysis=facial)) != 0:
-        raise SystemExit("FaceDetector.Init failed")
-    df = [(t.data_ptr(), f.shape[1], f.shape[0], 3, f.shape[1] * 3) for t, f in zip(dev, frames)]
+    return dev, [(t.data_ptr(), f.shape[1], f.shape[0], 3, f.shape[1] * 3) for t, f in zip(dev, frames)]
+
+
+def timed_batches(torch, det, df, passes):
+    """Best of `passes` DetectBatch calls (after one untimed call): (seconds, results)."""
     det.DetectBatch(device_frames=df)
-    best = None
-    faces = 0
-    for _ in range(steps):
+    torch.cuda.synchronize()
+    best, out = None, None
+    for _ in range(passes):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out = det.DetectBatch(device_frames=df)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        faces = sum(n for n, _ in out)
         best = dt if best is None else min(best, dt)
-    return {"fps": round(len(frames) / best, 2), "frames": len(frames), "faces": faces,
-            "sizes": sorted({"%dx%d" % (f.shape[1], f.shape[0]) for f in frames}),
-            "min_face": cfg["min_face"], "scale_factor": cfg["scale_factor"]}
+    return best, [f for _, f in out]
+
+
+def detector(api, W, H, min_face, max_face, sf, precision=0, mode=1, facial=False):
+    det = api.FaceDetector()
+    if det.Init(api.Param(max_image_size=(W, H), min_face_height=min_face, max_face_height=max_face,
+                          scale_factor=sf, facial_analysis=facial)) != 0:
+        raise SystemExit("FaceDetector.Init failed")
+    det.set_compute_precision(precision)
+    det.set_batch_mode(mode)
+    return det
+
+
+def secondary_fps(api, torch, names, per_name, steps, facial=False, precisions=(0,), mode=1):
+    """Frames/s of DetectBatch over frames of the named workloads (resident in GPU memory,
+    one detector initialised for the largest frame), best of `steps` passes, per compute
+    precision; recall of each reduced precision against FP32."""
+    from compactcnncascade_amd import recall
+    frames = []
+    for name in names:
+        frames.extend(make_frames(name, per_name))
+    dev, df = to_device(torch, frames)
+    cfg = synth.WORKLOADS[names[-1]]
+    W = max(f.shape[1] for f in frames)
+    H = max(f.shape[0] for f in frames)
+    out = {"frames": len(frames), "sizes": sorted({"%dx%d" % (f.shape[1], f.shape[0]) for f in frames}),
+           "min_face": cfg["min_face"], "scale_factor": cfg["scale_factor"]}
+    base = None
+    for p in precisions:
+        det = detector(api, W, H, cfg["min_face"], cfg["max_face"], cfg["scale_factor"], p, mode, facial)
+        dt, res = timed_batches(torch, det, df, steps)
+        key = ["fp32", "fp16_bf16", "fp16_fp8"][p]
+        leg = {"fps": round(len(frames) / dt, 2), "faces": int(sum(len(r) for r in res))}
+        if mode == 1:
+            leg["launches_per_batch"] = det.batch_stats()[0]
+        if p == 0:
+            base = res
+        elif base is not None:
+            rc, paired, nref, ntest = recall.recall(base, res)
+            leg["recall_vs_fp32"] = round(rc, 5)
+            leg["paired"] = [paired, nref, ntest]
+        out[key] = leg
+    out["fps"] = out["fp32"]["fps"] if "fp32" in out else None
+    del dev
+    return out
+
+
+def skip_leg(api, torch, W, H, n, steps):
+    """Learned tile skipping on held-out frames (faces >= 20, scale 1.15): the plain FP32
+    batch, skipping with the committed network at FP32 and at FP16/BF16, recall of each
+    against the plain cascade, tiles run."""
+    from compactcnncascade_amd import recall, skipnet
+    try:
+        params, meta = skipnet.load()
+    except OSError as e:
+        return {"unavailable": str(e)}
+    frames = held_out_frames(W, H, n, 900000)
+    dev, df = to_device(torch, frames)
+    out = {"frames": n, "frame": "%dx%d" % (W, H), "min_face": 20, "scale_factor": 1.15,
+           "network": {k: meta[k] for k in ("frames", "epochs", "slots") if k in meta}}
+    base = None
+    t = meta.get("threshold", 0.05)
+    for key, p, thr in (("plain_fp32", 0, None), ("skip_fp32", 0, t), ("skip_fp16_bf16", 1, t),
+                        ("plain_fp16_bf16", 1, None)):
+        det = detector(api, W, H, 20, 0, 1.15, p, 1)
+        if thr is not None:
+            det.set_skipnet(params, thr, meta["min_face"], meta["scale_factor"])
+        dt, res = timed_batches(torch, det, df, steps)
+        leg = {"fps": round(n / dt, 2), "faces": int(sum(len(r) for r in res))}
+        if thr is not None:
+            tiles, kept = det.skip_stats()
+            leg.update(threshold=thr, tiles=tiles, tiles_run=kept, tiles_run_frac=round(kept / max(1, tiles), 4))
+        if base is None:
+            base = res
+        else:
+            rc, paired, nref, ntest = recall.recall(base, res)
+            leg["recall_vs_plain"] = round(rc, 5)
+            leg["paired"] = [paired, nref, ntest]
+        out[key] = leg
+    out["gain_over_plain_fp32"] = round(out["skip_fp16_bf16"]["fps"] / out["plain_fp32"]["fps"], 2)
+    del dev
+    return out
 
 
 def host_frame_fps(api, torch, frames, param):
@@ -118,53 +218,85 @@ def stage1_macs(levels):
     return l1, rest
 
 
-STAGE2_MAC = 278496
-STAGE3_MAC = 276672
-
-
-def check_evals(res):
-    """Stage-2 and stage-3 network evaluations the check kernel performed (per-candidate
-    counts recorded by the kernel)."""
+def check_evals(res, status):
+    """Stage-2 and stage-3 network evaluations: (those the reference performs -- of the
+    candidates its ordered replay checks, status 2 --, all the check kernel performed)."""
     if len(res) == 0:
-        return 0, 0
-    return int(res[:, 6].sum()), int(res[:, 7].sum())
+        return (0, 0), (0, 0)
+    ev = status == 2
+    return (int(res[ev, 6].sum()), int(res[ev, 7].sum())), (int(res[:, 6].sum()), int(res[:, 7].sum()))
 
 
-def cpu_baseline(frames, params, budget_s):
-    """The reference's CPU path compiled from its own sources at -O3 (oracle/_ref),
-    single thread, over as many of the workload's frames as fit in budget_s."""
-    try:
-        from oracle.bind import Ref, default_params
-        ref = Ref("O3")
-    except Exception as e:  # noqa: BLE001
-        return {"value": None, "unit": "frames/s", "cores": 1, "kind": "reference",
-                "sample": "unavailable: %s" % e}
+def _cpu_worker(args):
+    frames, params, budget = args
+    from oracle.bind import Ref, default_params
+    ref = Ref("O3")
     h, w = frames.shape[1:3]
     p = default_params(w, h, **params)
     done = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
+    while time.perf_counter() - t0 < budget:
         i = done % len(frames)
         ref.detect_loop(frames[i:i + 1], p, 1)
         done += 1
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "frames/s", "cores": 1, "kind": "reference",
-            "sample": "%d detections over the workload's first %d %dx%d frames (cycled), one thread, %.1f s"
-                      % (done, min(done, len(frames)), w, h, dt)}
+    return done, time.perf_counter() - t0
 
 
-def measured_traffic(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE and
-    WRITE_SIZE passes summarised under profiles/ (line `bytes_per_launch N`), or None."""
-    path = os.path.join(ROOT, "profiles", "%s_%s_hbm_traffic.txt" % (kernel, workload))
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:  # noqa: BLE001
+        pass
+    return None
+
+
+def cpu_baseline(frames, params, budget_s):
+    """The reference's CPU path compiled from its own sources at -O3 (oracle/_ref) on the
+    workload's frames: one thread (latency), then one process per core (throughput)."""
+    try:
+        from oracle.bind import Ref  # noqa: F401
+    except Exception as e:  # noqa: BLE001
+        return {"value": None, "unit": "frames/s", "cores": 1, "kind": "reference", "sample": "unavailable: %s" % e}
+    sub = frames[:8]
+    done, dt = _cpu_worker((sub, params, budget_s))
+    one = done / dt
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    workers = max(1, cores)
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        res = list(ex.map(_cpu_worker, [(sub[k % len(sub):k % len(sub) + 1], params, budget_s / 2)
+                                        for k in range(workers)]))
+    allc = sum(d / t for d, t in res)
+    return {"value": round(allc, 2), "unit": "frames/s", "cores": workers, "kind": "reference",
+            "one_core_fps": round(one, 2), "one_core_ms_per_frame": round(1e3 / one, 1),
+            "cpu_model": cpu_model(),
+            "sample": "reference CPU path (-O3 -ffast-math, single-threaded as its Linux build is) on the first %d "
+                      "workload frames: %d detections in %.1f s on one core; then one process per core (%d) for "
+                      "%.1f s each" % (len(sub), done, dt, workers, budget_s / 2)}
+
+
+def measured(path_name):
+    """Counter summaries under profiles/ (lines `key value`)."""
+    path = os.path.join(ROOT, "profiles", path_name)
+    out = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("bytes_per_launch"):
-                    return int(float(line.split()[1]))
+                parts = line.split()
+                if len(parts) == 2:
+                    try:
+                        out[parts[0]] = float(parts[1])
+                    except ValueError:
+                        pass
     except OSError:
         pass
-    return None
+    return out
 
 
 def calibrate_copy(mib):
@@ -180,6 +312,59 @@ def calibrate_copy(mib):
     print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 5}), flush=True)
 
 
+def pmc_summary(db, calib_db, counter, frames):
+    """Per-kernel summary of one rocprofv3 counter pass (a rocpd database): dispatches,
+    average counter value (KiB per dispatch) and, scaled by what the same counter reads
+    for the calibration copy (bench.py --calibrate-copy 256: 256 MiB read and written per
+    dispatch), bytes per dispatch and per frame of the pass."""
+    import sqlite3
+
+    def per_kernel(path):
+        con = sqlite3.connect(path)
+        rows = con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                           (counter,)).fetchall()
+        agg = {}
+        for name, v in rows:
+            a = agg.setdefault(name.split("(")[0], [0, 0.0])
+            a[0] += 1
+            a[1] += float(v)
+        return agg
+
+    cal = per_kernel(calib_db)
+    copy = max(cal.items(), key=lambda kv: kv[1][1] / kv[1][0])
+    copy_kib = copy[1][1] / copy[1][0]
+    factor = (256 * 1024) / copy_kib
+    agg = per_kernel(db)
+    total = 0.0
+    print("# %s per dispatch; calibration: %s reads %.0f KiB for 262144 KiB moved -> factor %.4f"
+          % (counter, copy[0], copy_kib, factor))
+    for name, (n, v) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        b = v / n * factor * 1024
+        total += b * n
+        print("kernel %-50s dispatches %6d avg_kib %12.1f bytes_per_dispatch %14.0f" % (name[:50], n, v / n, b))
+    print("bytes_total %.0f" % total)
+    print("bytes_per_frame %.0f" % (total / max(1, frames)))
+
+
+def profile_pass(n, mode):
+    """Only the detection of the first n workload frames, one call per pass (for
+    rocprofv3 counter runs): mode 'single' = one Detect per frame, 'batch' = one
+    DetectBatch (graph) over the n frames."""
+    import torch
+    from compactcnncascade_amd import api
+    cfg = synth.WORKLOADS["fhd_min40_max240"]
+    frames = make_frames("fhd_min40_max240", n)
+    dev, df = to_device(torch, frames)
+    det = detector(api, cfg["w"], cfg["h"], cfg["min_face"], cfg["max_face"], cfg["scale_factor"])
+    if mode == "batch":
+        det.DetectBatch(device_frames=df)
+    else:
+        for (p, w, h, c, s) in df:
+            det.DetectDevice(ctypes.c_void_p(p), w, h, c, s)
+    torch.cuda.synchronize()
+    print(json.dumps({"profile_pass": n, "mode": mode}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,14 +372,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--frames", type=int, default=64)
     ap.add_argument("--workload", default="fhd_min40_max240")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="skip the 4K and mixed-size batch measurements reported beside `value`")
+                    help="skip the measurements reported beside `value`")
     ap.add_argument("--facial", action="store_true",
                     help="run the workload with facial analysis (the line's config says so)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
+                    help="only detect the first N workload frames (for rocprofv3 counter runs) and exit")
+    ap.add_argument("--profile-mode", default="single", choices=["single", "batch"])
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("DB", "CALIB_DB", "COUNTER", "FRAMES"),
+                    help="summarize a rocprofv3 counter database per kernel and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path here (frames would shard "
@@ -204,38 +394,42 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if args.pmc_summary:
+        db, cal, counter, nfr = args.pmc_summary
+        pmc_summary(db, cal, counter, int(nfr))
+        return
+    if args.profile_pass:
+        profile_pass(args.profile_pass, args.profile_mode)
+        return
 
     import torch
+    torch.zeros(1, device="cuda:0")
     from compactcnncascade_amd import api
 
     cfg = synth.WORKLOADS[args.workload]
     W, H = cfg["w"], cfg["h"]
     frames = make_frames(args.workload, args.frames)
-    dev = torch.device("cuda:0")
-    d_frames = torch.from_numpy(frames).to(dev)
+    d_frames = torch.from_numpy(frames).to("cuda:0")
     torch.cuda.synchronize()
     base = d_frames.data_ptr()
     fbytes = W * H * 3
-
-    det = api.FaceDetector()
+    dframes = [(base + i * fbytes, W, H, 3, W * 3) for i in range(args.frames)]
     param = api.Param(max_image_size=(W, H), min_face_height=cfg["min_face"], max_face_height=cfg["max_face"],
                       scale_factor=cfg["scale_factor"], facial_analysis=args.facial)
+
+    det = api.FaceDetector()
     if det.Init(param) != 0:
         raise SystemExit("FaceDetector.Init failed")
-
-    dframes = [(base + i * fbytes, W, H, 3, W * 3) for i in range(args.frames)]
-
-    def one_pass():
-        out = det.DetectBatch(device_frames=dframes)
-        return sum(n for n, _ in out)
-
-    # record of the work per frame (untimed): candidates and stage-2/3 evaluations
+    # record of the work per frame (untimed): candidates, replay status, stage-2/3 evaluations
     records = []
     for i in range(args.frames):
         k, _ = det.DetectDevice(ctypes.c_void_p(base + i * fbytes), W, H, 3, W * 3)
         if k < 0:
             raise SystemExit("Detect failed")
-        records.append(det.last_trace())
+        tr = det.last_trace()
+        tr["status"] = det.last_status()
+        records.append(tr)
+    progress("recorded %d frames" % args.frames)
     # single-frame pass: one Detect call at a time, so each frame's kernels have the GPU
     # to themselves; their durations (HIP events stamped by the launches) give the
     # per-kernel rooflines
@@ -244,53 +438,68 @@ def main():
         det.DetectDevice(ctypes.c_void_p(base + i * fbytes), W, H, 3, W * 3)
     tm = det.last_timing()
     det.set_profiling(False)
-    # the same pass without the events, timed as sync_single_frame_fps (not part of `value`)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.frames):
         det.DetectDevice(ctypes.c_void_p(base + i * fbytes), W, H, 3, W * 3)
     sync_fps = args.frames / (time.perf_counter() - t0)
 
-    # the timed region: K passes of pipelined batch detection over the frames
-    for _ in range(args.warmup):
-        one_pass()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    faces = 0
-    for _ in range(args.steps):
-        faces += one_pass()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    # the timed region: K DetectBatch passes over the frames, for each batch mode
+    modes = {}
+    for mode in (1, 0):
+        det.set_batch_mode(mode)
+        for _ in range(args.warmup):
+            det.DetectBatch(device_frames=dframes)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        faces = 0
+        for _ in range(args.steps):
+            out = det.DetectBatch(device_frames=dframes)
+            faces += sum(n for n, _ in out)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        modes[mode] = (dt, faces)
+        progress("batch mode %d: %.1f frames/s" % (mode, args.steps * args.frames / dt))
+    best_mode = min(modes, key=lambda m: modes[m][0])
+    dt, faces = modes[best_mode]
     nframes = args.steps * args.frames
     fps = nframes / dt
 
     # algorithmic work of the recorded pass
     s1 = np.array([stage1_macs(tr["levels"]) for tr in records], np.float64).mean(0)
     s1_mac = s1.sum()
-    e2 = e3 = 0
+    ref_e = np.zeros(2)
+    all_e = np.zeros(2)
     cands = 0
     for tr in records:
-        a, b = check_evals(tr["res"])
-        e2 += a
-        e3 += b
+        r, a = check_evals(tr["res"], tr["status"])
+        ref_e += r
+        all_e += a
         cands += len(tr["cand"])
-    e2 /= len(records)
-    e3 /= len(records)
+    ref_e /= len(records)
+    all_e /= len(records)
     per_frame_ms = {k: tm["totals"][k] / max(1, tm["frames"])
                     for k in ("gray", "stage1_l1", "stage1", "gap", "check", "after_check")}
-    # algorithmic FLOP (2 per MAC) per launch of the three VALU kernels, and what their
-    # event-timed launches achieve
+    # algorithmic FLOP (2 per MAC) per launch of the three VALU kernels: stage 1 as the
+    # reference computes it, stages 2/3 for the evaluations the reference performs
+    # (the kernel also evaluates the candidates the replay drops, counted as executed)
     kflops = {"k_stage1_l1": 2.0 * s1[0], "k_stage1": 2.0 * s1[1],
-              "k_check": 2.0 * (e2 * STAGE2_MAC + e3 * STAGE3_MAC)}
+              "k_check": 2.0 * (ref_e[0] * STAGE2_MAC + ref_e[1] * STAGE3_MAC)}
+    executed = {"k_check": 2.0 * (all_e[0] * STAGE2_MAC + all_e[1] * STAGE3_MAC)}
     kms = {"k_stage1_l1": per_frame_ms["stage1_l1"], "k_stage1": per_frame_ms["stage1"],
            "k_check": per_frame_ms["check"]}
-    kernels = {k: {"ms": round(kms[k], 4), "gflop": round(kflops[k] / 1e9, 4),
-                   "tflops": round(kflops[k] / (kms[k] * 1e-3) / 1e12, 3),
-                   "frac_of_exact_valu": round(kflops[k] / (kms[k] * 1e-3) / 1e12 / VALU_EXACT_PEAK_TOPS, 3)}
-               for k in kflops}
+    kernels = {}
+    for k in kflops:
+        tf = kflops[k] / (kms[k] * 1e-3) / 1e12
+        kernels[k] = {"ms": round(kms[k], 4), "gflop": round(kflops[k] / 1e9, 4), "tflops": round(tf, 3),
+                      "frac_of_fp32_peak": round(tf / FP32_PEAK_TFLOPS, 4),
+                      "frac_of_exact_valu": round(tf / VALU_EXACT_PEAK_TOPS, 3)}
+        if k in executed:
+            kernels[k]["gflop_executed"] = round(executed[k] / 1e9, 4)
     dominant = max(kms, key=lambda k: kms[k])
     achieved = kflops[dominant] / (kms[dominant] * 1e-3) / 1e12
-    in_bytes = W * H * 3
+    traffic = measured("%s_%s_hbm_traffic.txt" % (dominant, args.workload)).get("bytes_per_launch")
+    frame_bytes = measured("frame_%s_hbm_traffic.txt" % args.workload)
     line = {
         "metric": "1080p frames/sec (plain FP32 cascade, bit-exact with the reference CPU path)",
         "value": round(fps, 2),
@@ -307,47 +516,66 @@ def main():
         "config": {"workload": args.workload, "frames_per_step": args.frames, "frame": "%dx%d BGR u8" % (W, H),
                    "scale_factor": cfg["scale_factor"], "min_face": cfg["min_face"], "max_face": cfg["max_face"],
                    "facial_analysis": args.facial,
-                   "launch": "per-frame launches, consecutive frames pipelined over three streams, "
-                             "frames resident in HBM"},
+                   "launch": ("batch graph: one captured HIP graph launch per 64-frame batch" if best_mode == 1 else
+                              "per-frame launches pipelined over three streams") + ", frames resident in HBM"},
+        "batch_modes_fps": {"graph": round(nframes / modes[1][0], 2), "pipelined": round(nframes / modes[0][0], 2)},
         "roofline": {
             "kernel": dominant,
-            "bound": "mfma",
+            "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": measured_traffic(dominant, args.workload),
+            "frac_exact_valu": round(achieved / VALU_EXACT_PEAK_TOPS, 4),
+            "exact_valu_peak": VALU_EXACT_PEAK_TOPS,
+            "traffic": int(traffic) if traffic else None,
             "measured_in": "single-frame pass (one frame's kernels alone on the GPU), HIP start/stop events "
                            "stamped by the kernel launches",
-            "note": "FP32 VALU kernel on the exact path (separate mul/add, no packed FP32); its own ceiling "
-                    "is %.1f T ops/s, frac of that %.3f" % (VALU_EXACT_PEAK_TOPS, achieved / VALU_EXACT_PEAK_TOPS),
+            "note": "FP32 VALU kernel on the exact path (every multiply and add rounded separately: no FMA, no "
+                    "packed FP32), so its own ceiling is %.1f T ops/s; achieved counts 2 FLOP per MAC of the "
+                    "stage-2/3 evaluations the reference performs" % VALU_EXACT_PEAK_TOPS,
         },
         "kernel_ms_per_frame": {k: round(v, 4) for k, v in per_frame_ms.items()},
         "valu_kernels": kernels,
         "work_per_frame": {"stage1_gmac": round(s1_mac / 1e9, 4), "candidates": round(cands / len(records), 1),
-                           "stage2_evals": round(e2, 1), "stage3_evals": round(e3, 1)},
+                           "stage2_evals_reference": round(ref_e[0], 1), "stage3_evals_reference": round(ref_e[1], 1),
+                           "stage2_evals_executed": round(all_e[0], 1), "stage3_evals_executed": round(all_e[1], 1)},
         "faces_per_frame": round(faces / nframes, 2),
         "sync_single_frame_fps": round(sync_fps, 1),
-        "hbm_input_gbs": round(in_bytes * fps / 1e9, 3),
     }
+    if frame_bytes.get("bytes_per_frame"):
+        line["hbm"] = {"bytes_per_frame": int(frame_bytes["bytes_per_frame"]),
+                       "achieved_gbs": round(frame_bytes["bytes_per_frame"] * fps / 1e9, 1),
+                       "frac_of_peak": round(frame_bytes["bytes_per_frame"] * fps / 1e9 / HBM_PEAK_GBS, 4),
+                       "source": "profiles/frame_%s_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE of every "
+                                 "kernel of the single-frame pass, copy-calibrated)" % args.workload}
     if not args.no_secondary:
-        # the metric's 4K leg and the mixed-size batch (not part of `value`)
-        line["uhd_3840x2160"] = secondary_fps(api, torch, ["mixed_uhd"], 8, 3)
-        line["mixed_batch"] = secondary_fps(api, torch, ["mixed_vga", "mixed_hd", "mixed_fhd", "mixed_uhd"], 8, 3)
-        # the reference's CPU-runnable case (640x480, faces >= 40) and the deep pyramid
-        # (4800x2400, scale 1.05, faces >= 15: 91 levels)
-        line["vga_640x480"] = secondary_fps(api, torch, ["vga_min40"], 64, 3)
-        line["deep_4800x2400"] = secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2)
-        # the headline workload with facial analysis (landmarks network on accepted windows)
-        line["fhd_facial"] = secondary_fps(api, torch, ["fhd_min40_max240"], 64, 3, facial=True)
-        # the headline frames handed over in host memory: the upload crosses PCIe
-        line["fhd_host_frames"] = host_frame_fps(api, torch, frames, param)
+        # the metric's 4K leg, the mixed-size batch and the deep pyramid, each in FP32 and
+        # the reduced precisions (recall against FP32); the reference's CPU-runnable case;
+        # learned tile skipping; facial analysis; host frames over PCIe
+        legs = [
+            ("uhd_3840x2160", lambda: secondary_fps(api, torch, ["mixed_uhd"], 8, 3, precisions=(0, 1, 2))),
+            ("mixed_batch", lambda: secondary_fps(api, torch, ["mixed_vga", "mixed_hd", "mixed_fhd", "mixed_uhd"],
+                                                  8, 3, precisions=(0, 1, 2))),
+            ("deep_4800x2400", lambda: secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2,
+                                                     precisions=(0, 1, 2))),
+            ("vga_640x480", lambda: secondary_fps(api, torch, ["vga_min40"], 64, 3)),
+            ("skip_fhd_min20", lambda: skip_leg(api, torch, 1920, 1080, 32, 3)),
+            ("skip_uhd_min20", lambda: skip_leg(api, torch, 3840, 2160, 8, 3)),
+            ("fhd_facial", lambda: secondary_fps(api, torch, ["fhd_min40_max240"], 64, 3, facial=True)),
+            ("fhd_host_frames", lambda: host_frame_fps(api, torch, frames, param)),
+        ]
+        for name, leg in legs:
+            line[name] = leg()
+            progress("leg %s done" % name)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(frames, dict(min_face=cfg["min_face"], max_face=cfg["max_face"],
                                        scale_factor=cfg["scale_factor"]), args.cpu_seconds)
         line["cpu_baseline"] = cb
+        progress("cpu baseline done")
         if cb["value"]:
-            line["gpu_over_cpu"] = round(fps / cb["value"], 1)
+            line["gpu_over_cpu_all_cores"] = round(fps / cb["value"], 1)
+            line["gpu_over_cpu_one_core"] = round(fps / cb["one_core_fps"], 1)
     print(json.dumps(line), flush=True)
 
 

@@ -92,6 +92,9 @@ def load():
     L.ccnn_batch_stats.argtypes = [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]
     L.ccnn_set_tile_shape.argtypes = [_P, _I]
     L.ccnn_set_compute_precision.argtypes = [_P, _I]
+    L.ccnn_last_status.argtypes = [_P, _P, _I]
+    L.ccnn_set_skipnet.argtypes = [_P, _P, _F, _I, _F]
+    L.ccnn_skip_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.ccnn_get_compute_precision.argtypes = [_P]
     L.ccnn_stage1_ex.argtypes = [_P, _I, _I, _I, _P]
     L.ccnn_stage23_ex.argtypes = [_I, _P, _I, _I, _P]
@@ -272,6 +275,32 @@ class FaceDetector:
 
     def get_compute_precision(self):
         return self._L.ccnn_get_compute_precision(self._h)
+
+    def set_skipnet(self, params, threshold=0.05, min_face=20, scale_factor=1.15):
+        """Learned tile skipping on the batch path with a trained network (None: off)."""
+        if params is None:
+            rc = self._L.ccnn_set_skipnet(self._h, None, 0.5, 0, 0.0)
+        else:
+            self._skip = np.ascontiguousarray(params, np.float32)
+            rc = self._L.ccnn_set_skipnet(self._h, _ptr(self._skip), float(threshold), int(min_face),
+                                          float(np.float32(scale_factor)))
+        if rc != 0:
+            raise ValueError("ccnn_set_skipnet failed")
+
+    def skip_stats(self):
+        """(stage-1 response tiles of the last batch, tiles run)"""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self._L.ccnn_skip_stats(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def last_status(self):
+        """Replay status of the last frame's candidates (ccnn_last_status): 0 scan-dropped,
+        1 dropped before its check, 2 checked, 3 raw window."""
+        n = self._L.ccnn_last_status(self._h, None, 0)
+        out = np.zeros(max(n, 0), np.int32)
+        if n > 0:
+            self._L.ccnn_last_status(self._h, _ptr(out), n)
+        return out
 
     def set_tile_shape(self, shape):
         """Stage-1 tile shape id (0 .. 11) of the frames planned from now on."""

@@ -137,7 +137,7 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
     return d->det.set_skipnet(params, threshold, min_face, scale_factor);
 }
 
-int ccnn_skip_stats(const ccnn_detector* d, long long* tiles, long long* kept)
+int ccnn_skip_stats(ccnn_detector* d, long long* tiles, long long* kept)
 {
     if (!d || !tiles || !kept) return -1;
     d->det.skip_stats(tiles, kept);
@@ -295,6 +295,12 @@ long ccnn_last_responses(ccnn_detector* d, float* out, size_t cap)
 {
     if (!d) return -1;
     return d->det.last_responses(out, cap);
+}
+
+int ccnn_last_status(const ccnn_detector* d, int* out, int cap)
+{
+    if (!d) return -1;
+    return d->det.last_status(out, cap);
 }
 
 int ccnn_last_facial(const ccnn_detector* d, int* out, int cap)

@@ -17,9 +17,11 @@
 #include "ccnn_detector.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 namespace ccnn {
 
@@ -49,7 +51,7 @@ void Detector::release_batch()
     dfree(B.d_frames); dfree(B.d_pre); dfree(B.d_gray); dfree(B.d_s1); dfree(B.d_resp); dfree(B.d_levels);
     dfree(B.d_l1_order); dfree(B.d_ftab); dfree(B.d_counts); dfree(B.d_offs); dfree(B.d_meta); dfree(B.d_cands);
     dfree(B.d_s2); dfree(B.d_s3); dfree(B.d_snf); dfree(B.d_sn); dfree(B.d_keep);
-    hfree(B.h_ftab); hfree(B.h_meta); hfree(B.h_cands); hfree(B.h_s23); hfree(B.h_kept);
+    hfree(B.h_ftab); hfree(B.h_meta); hfree(B.h_cands); hfree(B.h_s23);
     if (B.st) { (void)hipStreamDestroy(B.st); B.st = nullptr; }
     const int gen = B.gen + 1;
     B = BatchArena();
@@ -234,10 +236,6 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
         if (grow(B.d_sn, B.sn_cap, (size_t)cells * (1 + 8 + 8 + CCNN_SKIP_SLOTS)) < 0 ||
             grow(B.d_keep, B.keep_cap, (size_t)(tiles + l1tiles)) < 0)
             return -1;
-        if (!B.h_kept) {
-            if (halloc(B.h_kept, B.hd_kept, 4) < 0) return -1;
-            moved = true;
-        }
     }
     (void)dummy_cap;
     if (moved) {
@@ -323,7 +321,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             float* z = a2 + 8 * cells;
             e = launch_skipnet(B.d_gray, B.d_ftab, B.d_snf, nf, (int)cells, d_skip_, x0, a1, a2, z, B.d_levels, nl,
                                (int)tiles, (int)l1tiles, seg[0].plan->shape, skip_threshold_, keep, l1_keep,
-                               B.hd_kept, B.st);
+                               B.st);
             launches += 6;
         }
         if (e == hipSuccess && tiles > 0) {
@@ -362,23 +360,28 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     HIP_OK(hipStreamSynchronize(B.st));
 
     B.tiles = tiles;
-    B.kept = skip && B.h_kept ? B.h_kept[0] : tiles;
+    B.skipped = skip && tiles > 0;
     const int total = B.h_meta[0];
     if (total < 0 || (size_t)total > B.cand_cap) return 1;
-    // each frame's candidates (in check order) through the decisions, the replay and Merger
+    // each frame's candidates (in check order) through the decisions, the replay and Merger;
+    // the frames are independent, so host threads take them in turn
     const Candidate* C = B.h_cands;
     const int2* S2 = B.h_s23;
     const int2* S3 = B.h_s23 + 2 * B.h_cap;
     const CheckParams P = check_params();
-    int c0 = 0;
-    std::vector<Candidate> lc;
-    std::vector<CheckResult> lr;
-    for (int fi = 0; fi < nf; ++fi) {
+    std::vector<int> cbeg(nf + 1, 0);
+    for (int fi = 0, c = 0; fi < nf; ++fi) {
+        cbeg[fi] = c;
+        while (c < total && C[c].level < lv_begin[fi + 1]) ++c;
+        cbeg[fi + 1] = c;
+    }
+    std::vector<Candidate> last_c;
+    std::vector<CheckResult> last_r;
+    std::vector<int> last_s;
+    auto run_frame = [&](int fi, std::vector<Candidate>& lc, std::vector<CheckResult>& lr, std::vector<int>* st) {
         const BatchFrame& f = seg[fi];
         const int l0 = lv_begin[fi], l1 = lv_begin[fi + 1];
-        int c1 = c0;
-        while (c1 < total && C[c1].level < l1) ++c1;
-        const int cnt = c1 - c0;
+        const int c0 = cbeg[fi], cnt = cbeg[fi + 1] - c0;
         lc.resize(cnt);
         lr.resize(checks ? cnt : 0);
         for (int k = 0; k < cnt; ++k) {
@@ -389,22 +392,47 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
                                        S3 + 2 * (size_t)(c0 + k));
         }
         const std::vector<Det> rects = replay(lv.data() + l0, l1 - l0, lc.data(), checks ? lr.data() : nullptr, cnt,
-                                              checks);
+                                              checks, st);
         emit_faces(rects, f.pre, faces[f.idx]);
-        if (fi == nf - 1) {
-            last_levels_ = f.plan->levels;
-            last_cands_ = lc;
-            last_results_ = lr;
-            last_count_ = cnt;
-            last_gray_ = B.d_gray + goff[fi];
-            last_gp_ = gpitch[fi];
-            last_gw_ = f.gw;
-            last_gh_ = f.gh;
-            last_resp_ = B.d_resp + roff[fi];
-        }
-        c0 = c1;
-    }
+    };
+    const int nthreads = std::max(1, std::min<int>({nf - 1, (int)std::thread::hardware_concurrency(), 8}));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        std::vector<Candidate> lc;
+        std::vector<CheckResult> lr;
+        for (int fi; (fi = next.fetch_add(1)) < nf - 1;) run_frame(fi, lc, lr, nullptr);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker);
+    // the last frame on this thread, with its record
+    run_frame(nf - 1, last_c, last_r, &last_s);
+    worker();
+    for (auto& t : pool) t.join();
+    const BatchFrame& fl = seg[nf - 1];
+    last_levels_ = fl.plan->levels;
+    last_cands_.swap(last_c);
+    last_results_.swap(last_r);
+    last_status_.swap(last_s);
+    last_count_ = (int)last_cands_.size();
+    last_gray_ = B.d_gray + goff[nf - 1];
+    last_gp_ = gpitch[nf - 1];
+    last_gw_ = fl.gw;
+    last_gh_ = fl.gh;
+    last_resp_ = B.d_resp + roff[nf - 1];
     return 0;
+}
+
+void Detector::skip_stats(long long* tiles, long long* kept)
+{
+    *tiles = *kept = batch_.tiles;
+    if (!batch_.skipped || !batch_.d_keep) return;
+    std::vector<unsigned char> h((size_t)batch_.tiles);
+    if (hipSetDevice(device_) != hipSuccess ||
+        hipMemcpy(h.data(), batch_.d_keep, h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return;
+    long long k = 0;
+    for (unsigned char c : h) k += c;
+    *kept = k;
 }
 
 int Detector::set_skipnet(const float* params, float threshold, int min_face, float scale_factor)

@@ -1012,7 +1012,7 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
     last_gw_ = s.gw;
     last_gh_ = s.gh;
     last_resp_ = s.d_resp;
-    const std::vector<Det> rects = replay(levels.data(), nl, cands, res, count, checks);
+    const std::vector<Det> rects = replay(levels.data(), nl, cands, res, count, checks, &last_status_);
     emit_faces(rects, s.rescale, faces);
     return (int)faces.size();
 }
@@ -1023,10 +1023,11 @@ int Detector::finish(Slot& s, std::vector<ccnn_face>& faces)
 // drop against this level's detections and the precomputed stage-2/3 outcome. Returns
 // the detections before post-processing.
 std::vector<Det> Detector::replay(const LevelDesc* levels, int nl, const Candidate* cands, const CheckResult* res,
-                                  int count, bool checks) const
+                                  int count, bool checks, std::vector<int>* status) const
 {
     std::vector<Det> rects;
     std::vector<int> points;
+    if (status) status->assign(count, 0);
     int ci = 0;
     for (int l = 0; l < nl; ++l) {
         const float scale = levels[l].scale;
@@ -1073,8 +1074,12 @@ std::vector<Det> Detector::replay(const LevelDesc* levels, int nl, const Candida
                             break;
                         }
                     }
-                    if (drop) continue;
+                    if (drop) {
+                        if (status) (*status)[pi] = 1;
+                        continue;
+                    }
                 }
+                if (status) (*status)[pi] = 2;
                 const CheckResult& r = res[pi];
                 // facial analysis drops the window, without the retry (:1749)
                 if (r.flag && set_.facial && !r.fa.ok) continue;
@@ -1092,6 +1097,7 @@ std::vector<Det> Detector::replay(const LevelDesc* levels, int nl, const Candida
         } else {
             for (int pi : points) {
                 const Candidate& c = cands[pi];
+                if (status) (*status)[pi] = 3;
                 Det d;
                 d.r = Rect((int)((float)(c.x * 4) * inv), (int)((float)(c.y * 4) * inv), (int)(23.f * inv),
                            (int)(27.f * inv));

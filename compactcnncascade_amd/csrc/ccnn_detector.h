@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <string>
 #include <vector>
@@ -41,7 +42,7 @@ hipError_t launch_resize_u8_tab(const FrameDesc* frames, int f, int dw, int dh, 
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
                           int ntiles, int nl1tiles, int shape, float threshold, unsigned char* keep,
-                          unsigned char* l1_keep, int* kept, hipStream_t st);
+                          unsigned char* l1_keep, hipStream_t st);
 // gray of every frame of a batch (FrameDesc table in device memory)
 hipError_t launch_gray_batch(const FrameDesc* frames, int nframes, int total_quads, float* gray, hipStream_t st);
 // ordered compaction of the windows above t1 over n responses (all levels of a batch back
@@ -238,8 +239,8 @@ struct BatchArena {
     SnFrame* d_snf = nullptr;     size_t snf_cap = 0;
     float* d_sn = nullptr;        size_t sn_cap = 0;       // x0, a1, a2, z back to back (1 + 8 + 8 + SLOTS per cell)
     unsigned char* d_keep = nullptr; size_t keep_cap = 0;  // response tiles, then L1 tiles
-    int* h_kept = nullptr;        int* hd_kept = nullptr;
-    long long tiles = 0, kept = 0;                         // of the last batch (kept = tiles without skipping)
+    long long tiles = 0;          // stage-1 response tiles of the last batch
+    bool skipped = false;         // whether the last batch ran the proposal network
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<long long> key;   // what the captured graph was built for
@@ -285,11 +286,9 @@ public:
     int precision() const { return precision_; }
     // Tile skipping (see ccnn_set_skipnet): params NULL switches it off.
     int set_skipnet(const float* params, float threshold, int min_face, float scale_factor);
-    void skip_stats(long long* tiles, long long* kept) const
-    {
-        *tiles = batch_.tiles;
-        *kept = batch_.kept;
-    }
+    // Response tiles of the last batch and how many of them ran (counted from the keep
+    // flags the batch left in device memory).
+    void skip_stats(long long* tiles, long long* kept);
     // Stage-1 tile shape of the plans made from now on (see TileShape); 0 or -1 for a bad id.
     int set_tile_shape(int id)
     {
@@ -328,6 +327,13 @@ public:
                     float* res);
     // facial analysis results of the last frame's candidates: {ok, gender, smile, glasses}
     int last_facial(int* out, int cap) const;
+    // replay status of the last frame's candidates (see replay)
+    int last_status(int* out, int cap) const
+    {
+        const int n = std::min(cap, (int)last_status_.size());
+        for (int i = 0; i < n && out; ++i) out[i] = last_status_[i];
+        return (int)last_status_.size();
+    }
 
 private:
     int init_device();
@@ -350,8 +356,10 @@ private:
     int finish(Slot& s, std::vector<ccnn_face>& faces);
     void release_slot(Slot& s);
     CheckParams check_params() const;
+    // status (optional): per candidate 0 dropped at the scan, 1 dropped before its check,
+    // 2 checked, 3 a raw window (no checks)
     std::vector<Det> replay(const LevelDesc* levels, int nl, const Candidate* cands, const CheckResult* res, int count,
-                            bool checks) const;
+                            bool checks, std::vector<int>* status = nullptr) const;
     void emit_faces(const std::vector<Det>& rects, bool rescale, std::vector<ccnn_face>& faces) const;
 
     bool empty_ = true;
@@ -388,6 +396,7 @@ private:
     const float* last_resp_ = nullptr;
     int last_count_ = 0;
     std::vector<Candidate> last_cands_;
+    std::vector<int> last_status_;
     std::vector<CheckResult> last_results_;
     std::vector<LevelDesc> last_levels_;
 

@@ -994,8 +994,9 @@ static_assert(12 * chk::P2N <= chk::IH * chk::IW && 24 * chk::RN + chk::RN <= ch
 // 2x2 max-pool is two lane swaps. The weights matrices are expanded per lane by the host
 // (MfmaW) and rounded to the operand type in the kernel. FP8 (e4m3) operands are scaled
 // by powers of two: the weights per layer (host), the L2 input per window (its largest
-// magnitude), and the products are scaled back exactly. L3, the hidden and output layers
-// run in FP32 as on the exact path.
+// magnitude), and the products are scaled back exactly; activations and weights enter as
+// two FP8 terms each (value and rounding error), three products per pair. L3, the hidden
+// and output layers run in FP32 as on the exact path.
 // ---------------------------------------------------------------------------------
 enum { RP_EXACT = 0, RP_BF16 = 1, RP_FP8 = 2 };
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -1029,11 +1030,64 @@ struct MOp<RP_FP8> {
         hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
         return (long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
     }
+    // the FP8 values of an operand, back in FP32
+    static __device__ __forceinline__ void unpack(T t, float (&v)[8])
+    {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const int lo = (int)((unsigned long long)t & 0xffffffffull), hi = (int)((unsigned long long)t >> 32);
+        const f2 a = __builtin_amdgcn_cvt_pk_f32_fp8(lo, false), b = __builtin_amdgcn_cvt_pk_f32_fp8(lo, true);
+        const f2 c = __builtin_amdgcn_cvt_pk_f32_fp8(hi, false), d = __builtin_amdgcn_cvt_pk_f32_fp8(hi, true);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = c.x; v[5] = c.y; v[6] = d.x; v[7] = d.y;
+    }
     static __device__ __forceinline__ f32x4_t mma(T a, T b, f32x4_t c)
     {
         return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
     }
 };
+
+// The weight operand of one lane: BF16 one term; FP8 two terms, the rounded value and its
+// rounding error, so that weight and activation each carry about seven significant bits.
+template <int RP>
+struct WOp {
+    typename MOp<RP>::T hi, lo;
+};
+template <int RP>
+__device__ __forceinline__ WOp<RP> pack_w(const float (&w)[8])
+{
+    using Op = MOp<RP>;
+    WOp<RP> r;
+    r.hi = Op::pack(w);
+    if constexpr (RP == RP_FP8) {
+        float v[8];
+        Op::unpack(r.hi, v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = w[j] - v[j];
+        r.lo = Op::pack(v);
+    } else {
+        r.lo = r.hi;
+    }
+    return r;
+}
+
+// The activation operand times the weights. FP8: the activations as the sum of two FP8
+// terms too (exact for the 8-bit pixels of L1), three products into one accumulator
+// (hi x hi, lo x hi, hi x lo; lo x lo is below the accumulator's rounding); BF16: one.
+template <int RP>
+__device__ __forceinline__ f32x4_t mma_act(const float (&a)[8], const WOp<RP>& b, f32x4_t acc)
+{
+    using Op = MOp<RP>;
+    const typename Op::T ah = Op::pack(a);
+    acc = Op::mma(ah, b.hi, acc);
+    if constexpr (RP == RP_FP8) {
+        float r[8];
+        Op::unpack(ah, r);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = a[j] - r[j];
+        acc = Op::mma(Op::pack(r), b.hi, acc);
+        acc = Op::mma(ah, b.lo, acc);
+    }
+    return acc;
+}
 
 // 2^e as a float (|e| < 126)
 __device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
@@ -1049,19 +1103,19 @@ __device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const 
     const int row = lane & 15, kb = lane >> 4;
     // this lane's weight operands: L1 (two column tiles), L2 (three sum pairs)
     float wv[8];
-    typename Op::T b1[2], b2[3];
+    WOp<RP> b1[2], b2[3];
     const float sw1 = RP == RP_FP8 ? mw[MfmaW::oScale] : 1.f, sw2 = RP == RP_FP8 ? mw[MfmaW::oScale + 1] : 1.f;
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) wv[j] = mw[MfmaW::oL1 + (nt * 64 + lane) * 8 + j] * sw1;
-        b1[nt] = Op::pack(wv);
+        b1[nt] = pack_w<RP>(wv);
     }
 #pragma unroll
     for (int jp = 0; jp < 3; ++jp) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) wv[j] = mw[MfmaW::oL2 + (jp * 64 + lane) * 8 + j] * sw2;
-        b2[jp] = Op::pack(wv);
+        b2[jp] = pack_w<RP>(wv);
     }
     // L1: 33 row tiles of 16 pooled positions over the 4 waves
     const float inv1 = 1.f / sw1;
@@ -1074,11 +1128,10 @@ __device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const 
             const int t = kb * 8 + j;                        // sample t of the 5x5 window
             a[j] = t < 25 ? in[(2 * py + t / 5) * IW + 2 * px + t % 5] : 0.f;
         }
-        const typename Op::T av = Op::pack(a);
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
             f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-            acc = Op::mma(av, b1[nt], acc);
+            acc = mma_act<RP>(a, b1[nt], acc);
             const int col = row + 16 * nt, m = col >> 2;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1139,7 +1192,7 @@ __device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const 
                 a[j] = src[(2 * qy + (t >> 2)) * P1W + 2 * qx + (t & 3)] * sx;
             }
             f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-            acc = Op::mma(Op::pack(a), b2[jp], acc);
+            acc = mma_act<RP>(a, b2[jp], acc);
             const int m = 4 * jp + (row >> 2);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {

@@ -8,11 +8,13 @@
 // (H8 x W8 = ceil(H/8) x ceil(W/8)); three 3x3 convolutions with zero padding ("same"),
 // 1 -> 8 -> 8 -> SLOTS channels, ReLU after the first two; output channel s = the logit
 // that pyramid level with scale index s (InitScales order) has an accepted window whose
-// centre lies in the 8x8 frame cell. Levels with scale index >= SLOTS are always run.
+// centre lies in the 8x8 frame cell. Levels with scale index >= SLOTS or scale < 0.5 are
+// always run.
 //
 // Kernels (FP32; the inference weights are the trained ones rounded to FP16):
 //   k_sn_down8      the network input of every frame of a batch from its gray image
 //   k_sn_conv       one 3x3 layer over a set of equally or differently sized frames
+//                   (one thread per position, every output channel)
 //   k_sn_tile_mask  keep flag of every stage-1 response tile of a batch
 //   k_sn_l1_mask    keep flag of every L1 tile: set when a kept response tile reads it
 //   k_sn_conv_bwd_data, k_sn_conv_bwd_w, k_sn_bce, k_sn_adam   training
@@ -42,11 +44,9 @@ __device__ __forceinline__ int sn_frame_of(const SnFrame* __restrict__ fr, int n
 // thread per cell. frames[f] = {h8, w8, off, pos_off} with off the frame's cell offset.
 __global__ void __launch_bounds__(256) k_sn_down8(const float* __restrict__ gray, const FrameDesc* __restrict__ ft,
                                                   const SnFrame* __restrict__ fr, int nf, int total,
-                                                  float* __restrict__ x0, int* __restrict__ kept)
+                                                  float* __restrict__ x0)
 {
     const int pos = blockIdx.x * 256 + threadIdx.x;
-    // the kept-tile counter that k_sn_tile_mask accumulates starts at zero
-    if (kept && pos == 0) *kept = 0;
     if (pos >= total) return;
     const int f = sn_frame_of(fr, nf, pos);
     const SnFrame F = fr[f];
@@ -64,39 +64,44 @@ __global__ void __launch_bounds__(256) k_sn_down8(const float* __restrict__ gray
 }
 
 // out[f][co][y][x] = relu?(b[co] + sum_{ci,ky,kx} w[co][ci][ky][kx] in[f][ci][y+ky-1][x+kx-1])
-// with zero padding. One thread per (frame position, output channel).
+// with zero padding. One thread per frame position computes every output channel from the
+// 9 * CIN inputs it loads once; the weights are read from LDS.
+template <int CIN, int COUT>
 __global__ void __launch_bounds__(256) k_sn_conv(const float* __restrict__ in, const float* __restrict__ w,
                                                  const float* __restrict__ b, float* __restrict__ out,
-                                                 const SnFrame* __restrict__ fr, int nf, int total, int cin, int cout,
-                                                 int relu)
+                                                 const SnFrame* __restrict__ fr, int nf, int total, int relu)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= total * cout) return;
-    const int co = t / total, pos = t - co * total;
+    __shared__ float s_w[COUT * CIN * 9 + COUT];
+    for (int i = threadIdx.x; i < COUT * CIN * 9; i += 256) s_w[i] = w[i];
+    for (int i = threadIdx.x; i < COUT; i += 256) s_w[COUT * CIN * 9 + i] = b[i];
+    __syncthreads();
+    const int pos = blockIdx.x * 256 + threadIdx.x;
+    if (pos >= total) return;
     const int f = sn_frame_of(fr, nf, pos);
     const SnFrame F = fr[f];
     const int c = pos - F.pos_off;
     const int y = c / F.w8, x = c - y * F.w8;
     const int plane = F.h8 * F.w8;
-    const float* src = in + (size_t)F.off * cin;
-    float acc = b[co];
-    for (int ci = 0; ci < cin; ++ci) {
-        const float* p = src + (size_t)ci * plane;
-        const float* k = w + (co * cin + ci) * 9;
+    const float* src = in + (size_t)F.off * CIN;
+    float v[CIN][9];
 #pragma unroll
-        for (int ky = 0; ky < 3; ++ky) {
-            const int yy = y + ky - 1;
-            if (yy < 0 || yy >= F.h8) continue;
+    for (int ci = 0; ci < CIN; ++ci)
 #pragma unroll
-            for (int kx = 0; kx < 3; ++kx) {
-                const int xx = x + kx - 1;
-                if (xx < 0 || xx >= F.w8) continue;
-                acc = __builtin_fmaf(k[ky * 3 + kx], p[yy * F.w8 + xx], acc);
-            }
+        for (int k = 0; k < 9; ++k) {
+            const int yy = y + k / 3 - 1, xx = x + k % 3 - 1;
+            v[ci][k] = (yy >= 0 && yy < F.h8 && xx >= 0 && xx < F.w8) ? src[(size_t)ci * plane + yy * F.w8 + xx] : 0.f;
         }
+    float* dst = out + (size_t)F.off * COUT + c;
+    for (int co = 0; co < COUT; ++co) {
+        float acc = s_w[COUT * CIN * 9 + co];
+        const float* k = s_w + co * CIN * 9;
+#pragma unroll
+        for (int ci = 0; ci < CIN; ++ci)
+#pragma unroll
+            for (int t = 0; t < 9; ++t) acc = __builtin_fmaf(k[ci * 9 + t], v[ci][t], acc);
+        if (relu) acc = fmaxf(acc, 0.f);
+        dst[(size_t)co * plane] = acc;
     }
-    if (relu) acc = fmaxf(acc, 0.f);
-    out[(size_t)F.off * cout + (size_t)co * plane + c] = acc;
 }
 
 // Gradient of the loss with respect to the layer input: gin[f][ci][y][x] =
@@ -230,13 +235,14 @@ __global__ void __launch_bounds__(256) k_sn_adam(float* __restrict__ p, const fl
 // Keep flag of every response tile of a batch: the largest logit of slot L.scl over the
 // frame cells that hold the centres of the tile's windows (level position 4 * x + 11.5,
 // 4 * y + 13.5, divided by the scale), widened by one cell, against the threshold's logit.
-// Levels whose scale index has no slot are kept.
+// Levels whose scale index has no slot are kept. One wave per tile: the lanes share the
+// tile's cells and reduce their maxima.
 __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restrict__ levels, int nlevels, int ntiles,
                                                       int TX, int TY, const float* __restrict__ logits,
                                                       const SnFrame* __restrict__ fr, float zt,
-                                                      unsigned char* __restrict__ keep, int* __restrict__ kept)
+                                                      unsigned char* __restrict__ keep)
 {
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= ntiles) return;
     int lo = 0, hi = nlevels - 1;
     while (lo < hi) {
@@ -245,7 +251,9 @@ __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restric
     }
     const LevelDesc& L = levels[lo];
     int k = 1;
-    if (L.scl < CCNN_SKIP_SLOTS) {
+    // levels of scale < 0.5 are cheap (their pixel count falls with the square of the
+    // scale) and their tiles span many cells: they always run
+    if (L.scl < CCNN_SKIP_SLOTS && L.scale >= 0.5f) {
         const int tt = t - L.tile_off;
         const int tx = tt % L.tiles_x, ty = tt / L.tiles_x;
         const SnFrame F = fr[L.frame];
@@ -257,13 +265,21 @@ __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restric
         const int v0 = max(0, (int)floorf((4.f * y0 + 13.5f) * inv) - 1);
         const int v1 = min(F.h8 - 1, (int)floorf((4.f * (y1 - 1) + 13.5f) * inv) + 1);
         const float* z = logits + (size_t)F.off * CCNN_SKIP_SLOTS + (size_t)L.scl * F.h8 * F.w8;
-        float mz = -INFINITY;
-        for (int v = v0; v <= v1; ++v)
-            for (int u = u0; u <= u1; ++u) mz = fmaxf(mz, z[v * F.w8 + u]);
-        k = mz >= zt;
+        const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
+        // a tile whose window centres fall outside the cell grid has no logit: it runs
+        if (nu > 0 && nv > 0) {
+            const int n = nu * nv;
+            float mz = -INFINITY;
+            for (int i = lane; i < n; i += 64) {
+                const int v = v0 + i / nu, u = u0 + i % nu;
+                mz = fmaxf(mz, z[v * F.w8 + u]);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mz = fmaxf(mz, __shfl_xor(mz, off, 64));
+            k = mz >= zt;
+        }
     }
-    keep[t] = (unsigned char)k;
-    if (kept && k) atomicAdd(kept, 1);
+    if (lane == 0) keep[t] = (unsigned char)k;
 }
 
 // Keep flag of every L1 tile: set when a kept response tile of its level reads it (the
@@ -347,9 +363,9 @@ struct SnRun {
     {
         const int T = total;
         auto blocks = [](long long k) { return (unsigned)((k + 255) / 256); };
-        k_sn_conv<<<blocks((long long)T * 8), 256>>>(x0.p, par.p + oW1, par.p + oB1, a1.p, fr.p, n, T, 1, 8, 1);
-        k_sn_conv<<<blocks((long long)T * 8), 256>>>(a1.p, par.p + oW2, par.p + oB2, a2.p, fr.p, n, T, 8, 8, 1);
-        k_sn_conv<<<blocks((long long)T * S), 256>>>(a2.p, par.p + oW3, par.p + oB3, z.p, fr.p, n, T, 8, S, 0);
+        k_sn_conv<1, 8><<<blocks(T), 256>>>(x0.p, par.p + oW1, par.p + oB1, a1.p, fr.p, n, T, 1);
+        k_sn_conv<8, 8><<<blocks(T), 256>>>(a1.p, par.p + oW2, par.p + oB2, a2.p, fr.p, n, T, 1);
+        k_sn_conv<8, S><<<blocks(T), 256>>>(a2.p, par.p + oW3, par.p + oB3, z.p, fr.p, n, T, 0);
         SN_OK(hipMemsetAsync(grad.p, 0, sizeof(float) * NPAR, nullptr));
         SN_OK(hipMemsetAsync(loss.p, 0, sizeof(float), nullptr));
         k_sn_bce<<<blocks((long long)T * S), 256>>>(z.p, y.p, dz.p, loss.p, T * S, pos_weight);
@@ -369,17 +385,17 @@ struct SnRun {
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
                           int ntiles, int nl1tiles, int shape, float threshold, unsigned char* keep,
-                          unsigned char* l1_keep, int* kept, hipStream_t st)
+                          unsigned char* l1_keep, hipStream_t st)
 {
     if (cells <= 0 || ntiles <= 0) return hipSuccess;
     const TileShape t = tile_shape(shape);
     const float zt = std::log(threshold / (1.f - threshold));
     auto blocks = [](long long k) { return (unsigned)((k + 255) / 256); };
-    k_sn_down8<<<blocks(cells), 256, 0, st>>>(gray, ft, fr, nf, cells, x0, kept);
-    k_sn_conv<<<blocks((long long)cells * 8), 256, 0, st>>>(x0, par + oW1, par + oB1, a1, fr, nf, cells, 1, 8, 1);
-    k_sn_conv<<<blocks((long long)cells * 8), 256, 0, st>>>(a1, par + oW2, par + oB2, a2, fr, nf, cells, 8, 8, 1);
-    k_sn_conv<<<blocks((long long)cells * S), 256, 0, st>>>(a2, par + oW3, par + oB3, z, fr, nf, cells, 8, S, 0);
-    k_sn_tile_mask<<<blocks(ntiles), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, zt, keep, kept);
+    k_sn_down8<<<blocks(cells), 256, 0, st>>>(gray, ft, fr, nf, cells, x0);
+    k_sn_conv<1, 8><<<blocks(cells), 256, 0, st>>>(x0, par + oW1, par + oB1, a1, fr, nf, cells, 1);
+    k_sn_conv<8, 8><<<blocks(cells), 256, 0, st>>>(a1, par + oW2, par + oB2, a2, fr, nf, cells, 1);
+    k_sn_conv<8, S><<<blocks(cells), 256, 0, st>>>(a2, par + oW3, par + oB3, z, fr, nf, cells, 0);
+    k_sn_tile_mask<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, zt, keep);
     k_sn_l1_mask<<<blocks(nl1tiles), 256, 0, st>>>(levels, nl, nl1tiles, t.ax, t.ay, t.tx, t.ty, keep, l1_keep);
     return hipGetLastError();
 }
@@ -418,9 +434,9 @@ extern "C" int ccnn_skipnet_forward(const float* params, const float* x0, int n,
     SN_OK(hipMemcpy(r.par.p, params, sizeof(float) * NPAR, hipMemcpyHostToDevice));
     SN_OK(hipMemcpy(r.x0.p, x0, sizeof(float) * P, hipMemcpyHostToDevice));
     const int T = r.total;
-    k_sn_conv<<<(T * 8 + 255) / 256, 256>>>(r.x0.p, r.par.p + oW1, r.par.p + oB1, r.a1.p, r.fr.p, n, T, 1, 8, 1);
-    k_sn_conv<<<(T * 8 + 255) / 256, 256>>>(r.a1.p, r.par.p + oW2, r.par.p + oB2, r.a2.p, r.fr.p, n, T, 8, 8, 1);
-    k_sn_conv<<<(T * S + 255) / 256, 256>>>(r.a2.p, r.par.p + oW3, r.par.p + oB3, r.z.p, r.fr.p, n, T, 8, S, 0);
+    k_sn_conv<1, 8><<<(T + 255) / 256, 256>>>(r.x0.p, r.par.p + oW1, r.par.p + oB1, r.a1.p, r.fr.p, n, T, 1);
+    k_sn_conv<8, 8><<<(T + 255) / 256, 256>>>(r.a1.p, r.par.p + oW2, r.par.p + oB2, r.a2.p, r.fr.p, n, T, 1);
+    k_sn_conv<8, S><<<(T + 255) / 256, 256>>>(r.a2.p, r.par.p + oW3, r.par.p + oB3, r.z.p, r.fr.p, n, T, 0);
     SN_OK(hipGetLastError());
     SN_OK(hipDeviceSynchronize());
     SN_OK(hipMemcpy(logits, r.z.p, sizeof(float) * S * P, hipMemcpyDeviceToHost));

@@ -7,7 +7,7 @@ per pyramid scale index s < SLOTS, the logit that the level of scale s has a win
 the cascade accepts whose centre lies in that cell. The labels come from the plain FP32
 cascade itself: every candidate window the stage-2/3 checks accept (the stage-1 response
 map thresholded at t1, then CPUCheckDetect; cnn_detector_v3.cpp:1485-1818, 1918-1947)
-marks its centre cell and the 8 cells around it.
+marks its centre cell and the cells within two of it.
 
 At detection time the batch path runs the network on every frame, keeps each stage-1
 response tile whose largest probability over its windows' cells reaches the threshold,
@@ -75,9 +75,9 @@ def scales_for(min_face, max_h, scale_factor, pattern_h=27):
     return out
 
 
-def labels_from_trace(shape, trace, scales):
+def labels_from_trace(shape, trace, scales, dilate=2):
     """uint8 labels [SLOTS, h8, w8] from one frame's trace (levels, candidates, check
-    results): accepted windows' centre cells and their 8 neighbours."""
+    results): accepted windows' centre cells and the cells within `dilate` of them."""
     h, w = shape
     h8, w8 = (h + 7) // 8, (w + 7) // 8
     y = np.zeros((SLOTS, h8, w8), np.uint8)
@@ -91,7 +91,7 @@ def labels_from_trace(shape, trace, scales):
         s = scales[scl]
         u = int(np.floor((4.0 * c[1] + 11.5) / s / 8.0))
         v = int(np.floor((4.0 * c[2] + 13.5) / s / 8.0))
-        y[scl, max(0, v - 1):min(h8, v + 2), max(0, u - 1):min(w8, u + 2)] = 1
+        y[scl, max(0, v - dilate):min(h8, v + dilate + 1), max(0, u - dilate):min(w8, u + dilate + 1)] = 1
     return y
 
 
@@ -150,7 +150,7 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def train(api, xs, ys, epochs=8, batch=64, lr=3e-3, pos_weight=20.0, seed=7, params=None):
+def train(api, xs, ys, epochs=8, batch=64, lr=3e-3, pos_weight=500.0, seed=7, params=None):
     """Trains on the GPU (Adam); returns (params, loss per epoch)."""
     L = _lib(api)
     p = np.ascontiguousarray(init_params(seed) if params is None else params, np.float32).copy()
@@ -204,11 +204,36 @@ def load(path=DEFAULT_MODEL):
         return np.array(z["params"], np.float32), json.loads(str(z["meta"]))
 
 
+def choose_threshold(api, params, frames, target=0.995, grid=(0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2, 0.15, 0.1,
+                                                               0.07, 0.05, 0.03, 0.02, 0.01)):
+    """The largest keep threshold whose detections on validation frames (1920x1080,
+    faces >= 20) pair with at least `target` of the plain cascade's (IoU >= 0.99), with
+    the tiles it runs; the sweep as a list of (threshold, recall, tiles run fraction)."""
+    from compactcnncascade_amd import recall
+    H, W = frames[0].shape[:2]
+    det = api.FaceDetector()
+    if det.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
+        raise RuntimeError("FaceDetector.Init failed")
+    base = [f for _, f in det.DetectBatch(images=frames, capacity=4096)]
+    sweep, best = [], None
+    for thr in grid:
+        det.set_skipnet(params, thr, MIN_FACE, SCALE_FACTOR)
+        got = [f for _, f in det.DetectBatch(images=frames, capacity=4096)]
+        tiles, kept = det.skip_stats()
+        rc = recall.recall(base, got)[0]
+        sweep.append((thr, round(rc, 5), round(kept / max(1, tiles), 4)))
+        if best is None and rc >= target:
+            best = thr
+    return (best if best is not None else grid[-1]), sweep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--frames", type=int, default=20000)
-    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--pos-weight", type=float, default=500.0)
+    ap.add_argument("--val-frames", type=int, default=24)
     ap.add_argument("--out", default=DEFAULT_MODEL)
     args = ap.parse_args()
     from compactcnncascade_amd import api
@@ -217,10 +242,16 @@ def main():
     print(json.dumps({"dataset_frames": args.frames, "faces": faces, "positive_fraction": float(ys.mean()),
                       "seconds": round(time.time() - t0, 1)}), flush=True)
     t1 = time.time()
-    p, losses = train(api, xs, ys, epochs=args.epochs)
+    p, losses = train(api, xs, ys, epochs=args.epochs, pos_weight=args.pos_weight)
+    train_s = round(time.time() - t1, 1)
+    del xs, ys
+    # validation frames: seeds 800000.. (training used 500000.., the tests and benchmark 900000..)
+    val = [synth.synth_frame(800000 + i, 1920, 1080, 10.0, 20, 300)[0] for i in range(args.val_frames)]
+    thr, sweep = choose_threshold(api, to_fp16(p), val)
     meta = {"min_face": MIN_FACE, "scale_factor": SCALE_FACTOR, "slots": SLOTS, "frames": args.frames,
-            "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "loss": [float(v) for v in losses],
-            "train_seconds": round(time.time() - t1, 1)}
+            "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "pos_weight": args.pos_weight,
+            "loss": [float(v) for v in losses], "train_seconds": train_s, "threshold": thr,
+            "validation": {"frames": args.val_frames, "frame": "1920x1080", "sweep": sweep}}
     save(args.out, p, meta)
     print(json.dumps(meta), flush=True)
 

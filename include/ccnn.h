@@ -150,8 +150,9 @@ int ccnn_autotune(ccnn_detector* d, int n, const ccnn_image* imgs, int device, i
 #define CCNN_SKIP_SLOTS 24
 #define CCNN_SKIP_PARAMS 2416
 int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int min_face, float scale_factor);
-/* Stage-1 response tiles in the last batch and how many of them ran. 0, or -1. */
-int ccnn_skip_stats(const ccnn_detector* d, long long* tiles, long long* kept);
+/* Stage-1 response tiles in the last batch and how many of them ran (read from the keep
+ * flags in device memory: a synchronous copy). 0, or -1. */
+int ccnn_skip_stats(ccnn_detector* d, long long* tiles, long long* kept);
 /* Training and checking of the proposal network on the GPU with the library's own
  * kernels (forward, backward, weighted binary cross-entropy, Adam). x0: n inputs of h8 x
  * w8 floats; labels: n x CCNN_SKIP_SLOTS x h8 x w8 bytes (0/1); loss = mean over all
@@ -229,7 +230,6 @@ int ccnn_stage1_ex(const float* level, int lw, int lh, int precision, float* out
  * (several host threads) race. 0, or -1. */
 #define CCNN_NOALLOC_SCRATCH_FLOATS (16u << 20)
 int ccnn_stage1_noalloc(const float* level, int lw, int lh, float* out);
-
 /* Stage-2/3 check of candidate windows on a gray image (CPUCheckDetect without the drop
  * logic): per candidate i, scale[i] and response-map position (x[i], y[i]); res[8*i..]
  * = {flag, mod, k1, k2, m1, m2, n2, n3}: acceptance, crop variant of the final evaluation,
@@ -262,6 +262,13 @@ int ccnn_stage_landmarks(const float* in, float* out);
  * NULL to query the counts. */
 int ccnn_last_trace(const ccnn_detector* d, int* n_levels, int* levels, int cap,
                     float* cand, float* res);
+
+/* How the reference's ordered replay (RunCheckDetect, cnn_detector_v3.cpp:1842-2011) treated
+ * each candidate of ccnn_last_trace, in the same order: 0 dropped at the scan
+ * (DropDetection), 1 dropped before its check (the level's drop, :1513-1533), 2 checked by
+ * stages 2/3, 3 a raw window (detect_mode disable). Returns the candidate count; out may
+ * be NULL. */
+int ccnn_last_status(const ccnn_detector* d, int* out, int cap);
 
 /* Stage-1 response maps of the frame that ccnn_last_trace describes: the levels in its
  * order, each a dense rh x rw map (ConvNeuralNetwork::Forward's response map of that

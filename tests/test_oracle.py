@@ -286,3 +286,23 @@ def test_oracle_sequences_match_reference_goldens(oracle, name):
     if name != "seq_setter_gray_after_colour":
         fresh = [oracle.detect(f, seq_params(over, mx))[0] for f in frames]
         assert any(len(a) != len(b) or not np.array_equal(a, b) for a, b in zip(out, fresh))
+
+
+# ---------------------------------------------------------------------------------
+# the FP64 oracle of the tile-skipping network
+# ---------------------------------------------------------------------------------
+def test_skipnet_oracle_gradient_matches_finite_differences():
+    from oracle import skipnet64 as sn
+    rng = np.random.default_rng(4)
+    p = sn.init_params(9)
+    x = rng.random((2, 6, 8)) - 0.5
+    y = rng.random((2, sn.S, 6, 8)) < 0.2
+    loss, g = sn.grad64(p, x, y, 4.0)
+    assert abs(loss - sn.loss64(p, x, y, 4.0)) < 1e-12
+    # every bias and a sample of the weights, central differences
+    idx = list(range(sn.NPAR - sn.S, sn.NPAR)) + list(rng.choice(sn.NPAR - sn.S, 40, replace=False))
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = 1e-6
+        fd = (sn.loss64(p + e, x, y, 4.0) - sn.loss64(p - e, x, y, 4.0)) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
