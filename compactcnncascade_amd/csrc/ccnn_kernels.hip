@@ -1093,7 +1093,7 @@ __device__ __forceinline__ f32x4_t mma_act(const float (&a)[8], const WOp<RP>& b
 __device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
 
 template <class N, int RP>
-__device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const int tid,
+__device__ __forceinline__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const int tid,
                                 const float* __restrict__ mw, int* s_red)
 {
     using namespace chk;
@@ -1208,7 +1208,7 @@ __device__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const 
 }
 
 template <class N, int RP = RP_EXACT>
-__device__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid,
+__device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid,
                               const float* __restrict__ mw = nullptr, int* s_red = nullptr)
 {
     using namespace chk;
