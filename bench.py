@@ -142,9 +142,11 @@ def secondary_fps(api, torch, names, per_name, steps, facial=False, precisions=(
 
 
 def skip_leg(api, torch, W, H, n, steps):
-    """Learned tile skipping on held-out frames (faces >= 20, scale 1.15): the plain FP32
-    batch, skipping with the committed network at FP32 and at FP16/BF16, recall of each
-    against the plain cascade, tiles run."""
+    """Learned tile skipping on held-out frames (faces >= 20, scale 1.15): the plain batch
+    at FP32 and FP16/BF16 against skipping with the committed network at its two
+    operating points ('recall': chosen on validation frames for recall >= 0.995 against
+    the plain cascade; 'fast': threshold 0.5), each at FP32 and FP16/BF16; recall of each
+    against the plain FP32 cascade, tiles run."""
     from compactcnncascade_amd import recall, skipnet
     try:
         params, meta = skipnet.load()
@@ -153,27 +155,32 @@ def skip_leg(api, torch, W, H, n, steps):
     frames = held_out_frames(W, H, n, 900000)
     dev, df = to_device(torch, frames)
     out = {"frames": n, "frame": "%dx%d" % (W, H), "min_face": 20, "scale_factor": 1.15,
-           "network": {k: meta[k] for k in ("frames", "epochs", "slots") if k in meta}}
+           "network": {k: meta[k] for k in ("frames", "epochs", "slots") if k in meta},
+           "operating_points": meta.get("operating_points")}
     base = None
-    t = meta.get("threshold", 0.05)
-    for key, p, thr in (("plain_fp32", 0, None), ("skip_fp32", 0, t), ("skip_fp16_bf16", 1, t),
-                        ("plain_fp16_bf16", 1, None)):
+    for key, p, op in (("plain_fp32", 0, None), ("skip_recall_fp32", 0, "recall"), ("skip_fast_fp32", 0, "fast"),
+                       ("plain_fp16_bf16", 1, None), ("skip_recall_fp16_bf16", 1, "recall"),
+                       ("skip_fast_fp16_bf16", 1, "fast")):
         det = detector(api, W, H, 20, 0, 1.15, p, 1)
-        if thr is not None:
-            det.set_skipnet(params, thr, meta["min_face"], meta["scale_factor"])
+        if op is not None:
+            if not meta.get("operating_points", {}).get(op):
+                continue
+            skipnet.apply_operating_point(det, params, meta, op)
         dt, res = timed_batches(torch, det, df, steps)
         leg = {"fps": round(n / dt, 2), "faces": int(sum(len(r) for r in res))}
-        if thr is not None:
+        if op is not None:
             tiles, kept = det.skip_stats()
-            leg.update(threshold=thr, tiles=tiles, tiles_run=kept, tiles_run_frac=round(kept / max(1, tiles), 4))
+            leg.update(tiles=tiles, tiles_run=kept, tiles_run_frac=round(kept / max(1, tiles), 4))
         if base is None:
             base = res
         else:
             rc, paired, nref, ntest = recall.recall(base, res)
-            leg["recall_vs_plain"] = round(rc, 5)
+            leg["recall_vs_plain_fp32"] = round(rc, 5)
             leg["paired"] = [paired, nref, ntest]
         out[key] = leg
-    out["gain_over_plain_fp32"] = round(out["skip_fp16_bf16"]["fps"] / out["plain_fp32"]["fps"], 2)
+    for key in ("skip_recall_fp16_bf16", "skip_fast_fp16_bf16"):
+        if key in out:
+            out["gain_" + key] = round(out[key]["fps"] / out["plain_fp32"]["fps"], 3)
     del dev
     return out
 
@@ -346,23 +353,52 @@ def pmc_summary(db, calib_db, counter, frames):
     print("bytes_per_frame %.0f" % (total / max(1, frames)))
 
 
-def profile_pass(n, mode):
-    """Only the detection of the first n workload frames, one call per pass (for
+def profile_pass(n, mode, workload="fhd_min40_max240", precision=0):
+    """Only the detection of the first n frames of a workload, one call per pass (for
     rocprofv3 counter runs): mode 'single' = one Detect per frame, 'batch' = one
-    DetectBatch (graph) over the n frames."""
+    DetectBatch (graph) over the n frames; precision as set_compute_precision."""
     import torch
     from compactcnncascade_amd import api
-    cfg = synth.WORKLOADS["fhd_min40_max240"]
-    frames = make_frames("fhd_min40_max240", n)
+    cfg = synth.WORKLOADS[workload]
+    frames = make_frames(workload, n)
     dev, df = to_device(torch, frames)
-    det = detector(api, cfg["w"], cfg["h"], cfg["min_face"], cfg["max_face"], cfg["scale_factor"])
+    det = detector(api, cfg["w"], cfg["h"], cfg["min_face"], cfg["max_face"], cfg["scale_factor"], precision)
     if mode == "batch":
         det.DetectBatch(device_frames=df)
     else:
         for (p, w, h, c, s) in df:
             det.DetectDevice(ctypes.c_void_p(p), w, h, c, s)
     torch.cuda.synchronize()
-    print(json.dumps({"profile_pass": n, "mode": mode}), flush=True)
+    print(json.dumps({"profile_pass": n, "mode": mode, "workload": workload, "precision": precision}), flush=True)
+
+
+def autotune_report(steps):
+    """The batch path's autotuner (ccnn_autotune: every stage-1 tile shape, then 2/3/4
+    persistent check workgroups per CU, each timed as whole DetectBatch passes) on the
+    mixed-size batch (8 each of 640x480, 1280x720, 1920x1080, 3840x2160, faces >= 20),
+    the 1080p workload (64 frames, faces 40..240) and 8 frames of 3840x2160 (faces >= 20):
+    one JSON line per workload."""
+    import torch
+    torch.zeros(1, device="cuda:0")
+    from compactcnncascade_amd import api
+    from compactcnncascade_amd.api import TILE_SHAPES
+    for name, names, per in (("mixed_batch", ["mixed_vga", "mixed_hd", "mixed_fhd", "mixed_uhd"], 8),
+                             ("fhd_min40_max240", ["fhd_min40_max240"], 64), ("uhd_min20", ["mixed_uhd"], 8)):
+        frames = []
+        for n in names:
+            frames.extend(make_frames(n, per))
+        dev, df = to_device(torch, frames)
+        cfg = synth.WORKLOADS[names[-1]]
+        W = max(f.shape[1] for f in frames)
+        H = max(f.shape[0] for f in frames)
+        det = detector(api, W, H, cfg["min_face"], cfg["max_face"], cfg["scale_factor"])
+        best, sms, gms = det.autotune(device_frames=df, reps=steps)
+        print(json.dumps({"autotune": name, "frames": len(frames), "best_shape": int(best),
+                          "best_shape_tiles": TILE_SHAPES[int(best)],
+                          "shape_ms": {"%d %s" % (i, TILE_SHAPES[i]): round(float(v), 4) for i, v in enumerate(sms)},
+                          "check_workgroups_per_cu_ms": {str(g): round(float(v), 4) for g, v in zip((2, 3, 4), gms)}}),
+              flush=True)
+        del dev
 
 
 def main():
@@ -383,6 +419,9 @@ def main():
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only detect the first N workload frames (for rocprofv3 counter runs) and exit")
     ap.add_argument("--profile-mode", default="single", choices=["single", "batch"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="only run the batch path's autotuner on the benchmark workloads and exit")
+    ap.add_argument("--profile-precision", type=int, default=0, choices=[0, 1, 2])
     ap.add_argument("--pmc-summary", nargs=4, metavar=("DB", "CALIB_DB", "COUNTER", "FRAMES"),
                     help="summarize a rocprofv3 counter database per kernel and exit")
     args = ap.parse_args()
@@ -398,8 +437,11 @@ def main():
         db, cal, counter, nfr = args.pmc_summary
         pmc_summary(db, cal, counter, int(nfr))
         return
+    if args.autotune:
+        autotune_report(max(1, args.steps))
+        return
     if args.profile_pass:
-        profile_pass(args.profile_pass, args.profile_mode)
+        profile_pass(args.profile_pass, args.profile_mode, args.workload, args.profile_precision)
         return
 
     import torch

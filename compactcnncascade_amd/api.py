@@ -23,6 +23,11 @@ DETECT_DISABLE, DETECT_SYNC, DETECT_ASYNC = 0, 1, 2
 PRECISION_DEF, PRECISION_LOW, PRECISION_NORMAL, PRECISION_HIGH, PRECISION_ULTRA = 0, 1, 2, 3, 4
 # compute precisions of this library (ccnn_set_compute_precision)
 FP32, FP16_BF16, FP16_FP8 = 0, 1, 2
+# stage-1 tile shapes by id (L1 tile, response tile), numbered as the library does:
+# id = L1 shape index * 3 + response shape index
+_L1_SHAPES = ((64, 16), (32, 16), (32, 32), (64, 8))
+_RESP_SHAPES = ((16, 16), (32, 8), (16, 8))
+TILE_SHAPES = ["L1 %dx%d, responses %dx%d" % (a + r) for a in _L1_SHAPES for r in _RESP_SHAPES]
 
 
 class CParam(ctypes.Structure):
@@ -95,6 +100,7 @@ def load():
     L.ccnn_last_status.argtypes = [_P, _P, _I]
     L.ccnn_set_skipnet.argtypes = [_P, _P, _F, _I, _F]
     L.ccnn_skip_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
+    L.ccnn_set_skip_window.argtypes = [_P, _I, _I]
     L.ccnn_get_compute_precision.argtypes = [_P]
     L.ccnn_stage1_ex.argtypes = [_P, _I, _I, _I, _P]
     L.ccnn_stage23_ex.argtypes = [_I, _P, _I, _I, _P]
@@ -286,6 +292,12 @@ class FaceDetector:
                                           float(np.float32(scale_factor)))
         if rc != 0:
             raise ValueError("ccnn_set_skipnet failed")
+
+    def set_skip_window(self, slot_radius=0, cell_margin=1):
+        """Keep window of the tile mask: the scale slots within slot_radius of a level's
+        own and the cells within cell_margin of its windows' centres."""
+        if self._L.ccnn_set_skip_window(self._h, int(slot_radius), int(cell_margin)) != 0:
+            raise ValueError("bad skip window")
 
     def skip_stats(self):
         """(stage-1 response tiles of the last batch, tiles run)"""

@@ -137,6 +137,12 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
     return d->det.set_skipnet(params, threshold, min_face, scale_factor);
 }
 
+int ccnn_set_skip_window(ccnn_detector* d, int slot_radius, int cell_margin)
+{
+    if (!d) return -1;
+    return d->det.set_skip_window(slot_radius, cell_margin);
+}
+
 int ccnn_skip_stats(ccnn_detector* d, long long* tiles, long long* kept)
 {
     if (!d || !tiles || !kept) return -1;

@@ -129,6 +129,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     const bool skip = d_skip_ && param.min_face_height == skip_min_face_ && param.scale_factor == skip_sf_;
     key.push_back(skip);
     key.push_back(skip ? (long long)(skip_threshold_ * 1e6f) : 0);
+    key.push_back(skip ? skip_srad_ * 64 + skip_margin_ : 0);
     for (const BatchFrame& f : seg)
         if (f.plan->shape != seg[0].plan->shape) return 1;   // one tile shape per launch
     size_t gray_n = 0, s1_n = 0, resp_n = 0, frames_n = 0, pre_n = 0;
@@ -320,8 +321,8 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             float* a2 = a1 + 8 * cells;
             float* z = a2 + 8 * cells;
             e = launch_skipnet(B.d_gray, B.d_ftab, B.d_snf, nf, (int)cells, d_skip_, x0, a1, a2, z, B.d_levels, nl,
-                               (int)tiles, (int)l1tiles, seg[0].plan->shape, skip_threshold_, keep, l1_keep,
-                               B.st);
+                               (int)tiles, (int)l1tiles, seg[0].plan->shape, skip_threshold_, skip_srad_,
+                               skip_margin_, keep, l1_keep, B.st);
             launches += 6;
         }
         if (e == hipSuccess && tiles > 0) {

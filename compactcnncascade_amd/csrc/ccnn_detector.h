@@ -41,8 +41,8 @@ hipError_t launch_resize_u8_tab(const FrameDesc* frames, int f, int dw, int dh, 
 // response tiles (keep) and L1 tiles (l1_keep); kept (optional) counts the kept tiles
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
-                          int ntiles, int nl1tiles, int shape, float threshold, unsigned char* keep,
-                          unsigned char* l1_keep, hipStream_t st);
+                          int ntiles, int nl1tiles, int shape, float threshold, int srad, int margin,
+                          unsigned char* keep, unsigned char* l1_keep, hipStream_t st);
 // gray of every frame of a batch (FrameDesc table in device memory)
 hipError_t launch_gray_batch(const FrameDesc* frames, int nframes, int total_quads, float* gray, hipStream_t st);
 // ordered compaction of the windows above t1 over n responses (all levels of a batch back
@@ -286,6 +286,16 @@ public:
     int precision() const { return precision_; }
     // Tile skipping (see ccnn_set_skipnet): params NULL switches it off.
     int set_skipnet(const float* params, float threshold, int min_face, float scale_factor);
+    // Keep window of the tile mask: scale slots within slot_radius of a level's own, cells
+    // within cell_margin of its windows' centres (defaults 0 and 1). 0, or -1.
+    int set_skip_window(int slot_radius, int cell_margin)
+    {
+        if (slot_radius < 0 || slot_radius >= CCNN_SKIP_SLOTS || cell_margin < 0 || cell_margin > 16) return -1;
+        skip_srad_ = slot_radius;
+        skip_margin_ = cell_margin;
+        batch_.key.clear();
+        return 0;
+    }
     // Response tiles of the last batch and how many of them ran (counted from the keep
     // flags the batch left in device memory).
     void skip_stats(long long* tiles, long long* kept);
@@ -375,6 +385,8 @@ private:
     float skip_threshold_ = 0.5f;
     int skip_min_face_ = 0;
     float skip_sf_ = 0.f;
+    int skip_srad_ = 0;
+    int skip_margin_ = 1;
     std::vector<float> scales_;
     // CNNDetector::num_scales: the scales RunCPUDetect visits. Init sets it to the scale
     // count, each frame's skip rule lowers it to the first scale whose level is too small,
@@ -410,7 +422,10 @@ private:
     int check_per_cu_ = 4;         // persistent k_check workgroups per CU on the batch path
     int batch_mode_ = 1;
     long long plan_ids_ = 0;
-    int shape_ = 0;                // stage-1 tile shape of new plans (set_tile_shape, autotune)
+    // stage-1 tile shape of new plans (set_tile_shape, autotune): 6 = L1 tiles of 32 x 32,
+    // response tiles of 16 x 16, the autotuner's pick on gfx950 for the 1080p, 4K and
+    // mixed-size batches (bench.py --autotune)
+    int shape_ = 6;
     int batch_captures_ = 0;
     BatchArena batch_;
     int fa_blocks_ = 64;           // k_facial workgroups (76 KB LDS each); windows dealt round-robin

@@ -232,14 +232,14 @@ __global__ void __launch_bounds__(256) k_sn_adam(float* __restrict__ p, const fl
     p[i] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-// Keep flag of every response tile of a batch: the largest logit of slot L.scl over the
-// frame cells that hold the centres of the tile's windows (level position 4 * x + 11.5,
-// 4 * y + 13.5, divided by the scale), widened by one cell, against the threshold's logit.
-// Levels whose scale index has no slot are kept. One wave per tile: the lanes share the
-// tile's cells and reduce their maxima.
+// Keep flag of every response tile of a batch: the largest logit of the slots within
+// `srad` of L.scl over the frame cells that hold the centres of the tile's windows (level
+// position 4 * x + 11.5, 4 * y + 13.5, divided by the scale), widened by `margin` cells,
+// against the threshold's logit. Levels whose scale index has no slot are kept. One wave
+// per tile: the lanes share the tile's (slot, cell) pairs and reduce their maxima.
 __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restrict__ levels, int nlevels, int ntiles,
                                                       int TX, int TY, const float* __restrict__ logits,
-                                                      const SnFrame* __restrict__ fr, float zt,
+                                                      const SnFrame* __restrict__ fr, float zt, int srad, int margin,
                                                       unsigned char* __restrict__ keep)
 {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -260,19 +260,22 @@ __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restric
         // responses of the tile: x in [x0, x1), y in [y0, y1)
         const int x0 = tx * TX, x1 = min(x0 + TX, L.rw), y0 = ty * TY, y1 = min(y0 + TY, L.rh);
         const float inv = 1.f / (8.f * L.scale);
-        const int u0 = max(0, (int)floorf((4.f * x0 + 11.5f) * inv) - 1);
-        const int u1 = min(F.w8 - 1, (int)floorf((4.f * (x1 - 1) + 11.5f) * inv) + 1);
-        const int v0 = max(0, (int)floorf((4.f * y0 + 13.5f) * inv) - 1);
-        const int v1 = min(F.h8 - 1, (int)floorf((4.f * (y1 - 1) + 13.5f) * inv) + 1);
-        const float* z = logits + (size_t)F.off * CCNN_SKIP_SLOTS + (size_t)L.scl * F.h8 * F.w8;
+        const int u0 = max(0, (int)floorf((4.f * x0 + 11.5f) * inv) - margin);
+        const int u1 = min(F.w8 - 1, (int)floorf((4.f * (x1 - 1) + 11.5f) * inv) + margin);
+        const int v0 = max(0, (int)floorf((4.f * y0 + 13.5f) * inv) - margin);
+        const int v1 = min(F.h8 - 1, (int)floorf((4.f * (y1 - 1) + 13.5f) * inv) + margin);
+        const int s0 = max(0, L.scl - srad), s1 = min(CCNN_SKIP_SLOTS - 1, L.scl + srad);
+        const size_t plane = (size_t)F.h8 * F.w8;
+        const float* z = logits + (size_t)F.off * CCNN_SKIP_SLOTS + (size_t)s0 * plane;
         const int nu = u1 - u0 + 1, nv = v1 - v0 + 1;
         // a tile whose window centres fall outside the cell grid has no logit: it runs
         if (nu > 0 && nv > 0) {
-            const int n = nu * nv;
+            const int n = nu * nv, ns = s1 - s0 + 1;
             float mz = -INFINITY;
-            for (int i = lane; i < n; i += 64) {
-                const int v = v0 + i / nu, u = u0 + i % nu;
-                mz = fmaxf(mz, z[v * F.w8 + u]);
+            for (int i = lane; i < n * ns; i += 64) {
+                const int sl = i / n, c = i - sl * n;
+                const int v = v0 + c / nu, u = u0 + c % nu;
+                mz = fmaxf(mz, z[sl * plane + (size_t)v * F.w8 + u]);
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) mz = fmaxf(mz, __shfl_xor(mz, off, 64));
@@ -384,8 +387,8 @@ struct SnRun {
 
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
-                          int ntiles, int nl1tiles, int shape, float threshold, unsigned char* keep,
-                          unsigned char* l1_keep, hipStream_t st)
+                          int ntiles, int nl1tiles, int shape, float threshold, int srad, int margin,
+                          unsigned char* keep, unsigned char* l1_keep, hipStream_t st)
 {
     if (cells <= 0 || ntiles <= 0) return hipSuccess;
     const TileShape t = tile_shape(shape);
@@ -395,7 +398,7 @@ hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame*
     k_sn_conv<1, 8><<<blocks(cells), 256, 0, st>>>(x0, par + oW1, par + oB1, a1, fr, nf, cells, 1);
     k_sn_conv<8, 8><<<blocks(cells), 256, 0, st>>>(a1, par + oW2, par + oB2, a2, fr, nf, cells, 1);
     k_sn_conv<8, S><<<blocks(cells), 256, 0, st>>>(a2, par + oW3, par + oB3, z, fr, nf, cells, 0);
-    k_sn_tile_mask<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, zt, keep);
+    k_sn_tile_mask<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, zt, srad, margin, keep);
     k_sn_l1_mask<<<blocks(nl1tiles), 256, 0, st>>>(levels, nl, nl1tiles, t.ax, t.ay, t.tx, t.ty, keep, l1_keep);
     return hipGetLastError();
 }

@@ -10,11 +10,19 @@ map thresholded at t1, then CPUCheckDetect; cnn_detector_v3.cpp:1485-1818, 1918-
 marks its centre cell and the cells within two of it.
 
 At detection time the batch path runs the network on every frame, keeps each stage-1
-response tile whose largest probability over its windows' cells reaches the threshold,
-and runs the plain cascade on those tiles only; the levels without a slot, and every
+response tile whose largest probability over its windows' cells (and nearby scale slots)
+reaches the threshold, and runs the plain cascade on those tiles only; the levels without a slot, and every
 tile the network does not rule out with that confidence, run as usual.
 
     python -m compactcnncascade_amd.skipnet train --frames 20000 --out models/skipnet_min20_sf115.npz
+    python -m compactcnncascade_amd.skipnet validate --model M --out M   (operating points only)
+    python -m compactcnncascade_amd.skipnet evaluate --model M --held-out 256
+
+Operating points, chosen on validation frames (1920x1080 and 3840x2160, seeds apart from
+training and testing) over keep windows (scale slots within a radius of a level's own,
+cells within a margin of its windows' centres) and thresholds: 'recall', the fewest
+tiles run at recall >= 0.995 against the plain cascade on every validation set, and
+'fast', threshold 0.5 with the narrowest window.
 """
 import argparse
 import ctypes
@@ -204,56 +212,178 @@ def load(path=DEFAULT_MODEL):
         return np.array(z["params"], np.float32), json.loads(str(z["meta"]))
 
 
-def choose_threshold(api, params, frames, target=0.995, grid=(0.9, 0.8, 0.7, 0.6, 0.5, 0.4, 0.3, 0.2, 0.15, 0.1,
-                                                               0.07, 0.05, 0.03, 0.02, 0.01)):
-    """The largest keep threshold whose detections on validation frames (1920x1080,
-    faces >= 20) pair with at least `target` of the plain cascade's (IoU >= 0.99), with
-    the tiles it runs; the sweep as a list of (threshold, recall, tiles run fraction)."""
+WINDOWS = ((0, 1), (1, 1), (2, 1), (0, 2), (1, 2), (2, 2))
+THRESHOLDS = (0.5, 0.3, 0.2, 0.1, 0.05, 0.03, 0.02, 0.01)
+
+
+def _gen_frame(args):
+    seed, w, h = args
+    return synth.synth_frame(seed, w, h, 10.0, 20, 300)[0]
+
+
+def validation_frames(n_fhd=48, n_uhd=12, workers=16):
+    """Validation frames: seeds 800000.. at 1920x1080 and 810000.. at 3840x2160 (training
+    uses seeds from 500000, the tests and the benchmark from 900000)."""
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        fhd = list(ex.map(_gen_frame, [(800000 + i, 1920, 1080) for i in range(n_fhd)]))
+        uhd = list(ex.map(_gen_frame, [(810000 + i, 3840, 2160) for i in range(n_uhd)]))
+    return {"1920x1080": fhd, "3840x2160": uhd}
+
+
+def sweep_operating_points(api, params, sets, windows=WINDOWS, thresholds=THRESHOLDS, chunk=16):
+    """Recall against the plain cascade (boxes paired at IoU >= 0.99) and the fraction of
+    stage-1 tiles run, per frame set, for every keep window (slot radius, cell margin)
+    and threshold: a list of {window, threshold, per set {recall, run}}."""
     from compactcnncascade_amd import recall
-    H, W = frames[0].shape[:2]
-    det = api.FaceDetector()
-    if det.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
-        raise RuntimeError("FaceDetector.Init failed")
-    base = [f for _, f in det.DetectBatch(images=frames, capacity=4096)]
-    sweep, best = [], None
-    for thr in grid:
-        det.set_skipnet(params, thr, MIN_FACE, SCALE_FACTOR)
-        got = [f for _, f in det.DetectBatch(images=frames, capacity=4096)]
-        tiles, kept = det.skip_stats()
-        rc = recall.recall(base, got)[0]
-        sweep.append((thr, round(rc, 5), round(kept / max(1, tiles), 4)))
-        if best is None and rc >= target:
-            best = thr
-    return (best if best is not None else grid[-1]), sweep
+    out = []
+    dets = {}
+    for name, frames in sets.items():
+        H, W = frames[0].shape[:2]
+        det = api.FaceDetector()
+        if det.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
+            raise RuntimeError("FaceDetector.Init failed")
+        base = []
+        for k in range(0, len(frames), chunk):
+            base += [f for _, f in det.DetectBatch(images=frames[k:k + chunk], capacity=4096)]
+        dets[name] = (det, base)
+    for (srad, margin) in windows:
+        for thr in thresholds:
+            row = {"slot_radius": srad, "cell_margin": margin, "threshold": thr}
+            for name, frames in sets.items():
+                det, base = dets[name]
+                det.set_skipnet(params, thr, MIN_FACE, SCALE_FACTOR)
+                det.set_skip_window(srad, margin)
+                got, tiles, kept = [], 0, 0
+                for k in range(0, len(frames), chunk):
+                    got += [f for _, f in det.DetectBatch(images=frames[k:k + chunk], capacity=4096)]
+                    a, b = det.skip_stats()
+                    tiles += a
+                    kept += b
+                rc = recall.recall(base, got)[0]
+                row[name] = {"recall": round(rc, 5), "run": round(kept / max(1, tiles), 4)}
+            out.append(row)
+    return out
+
+
+def choose_operating_points(sweep, target=0.995):
+    """'recall': the point that reaches `target` on every set with the fewest tiles run
+    (mean over the sets); 'fast': threshold 0.5 with the default window (0, 1)."""
+    sets = [k for k in sweep[0] if isinstance(sweep[0][k], dict)]
+    best = None
+    for row in sweep:
+        if all(row[k]["recall"] >= target for k in sets):
+            run = sum(row[k]["run"] for k in sets) / len(sets)
+            if best is None or run < best[0]:
+                best = (run, row)
+    pick = lambda r: {"threshold": r["threshold"], "slot_radius": r["slot_radius"], "cell_margin": r["cell_margin"]}
+    fast = next(r for r in sweep if r["threshold"] == 0.5 and r["slot_radius"] == 0 and r["cell_margin"] == 1)
+    return {"recall": pick(best[1]) if best else None, "fast": pick(fast), "target": target}
+
+
+def apply_operating_point(det, params, meta, which="recall"):
+    """Switches tile skipping on in a detector at one of the model's operating points."""
+    op = meta["operating_points"][which]
+    det.set_skipnet(params, op["threshold"], meta["min_face"], meta["scale_factor"])
+    det.set_skip_window(op["slot_radius"], op["cell_margin"])
+    return op
+
+
+def validate(api, params, meta, n_fhd=48, n_uhd=12):
+    """Sweeps the operating points of a trained network on the validation frames and
+    records the sweep and the chosen points in its meta."""
+    sets = validation_frames(n_fhd, n_uhd)
+    sweep = sweep_operating_points(api, to_fp16(params), sets)
+    meta["operating_points"] = choose_operating_points(sweep)
+    meta["validation"] = {"frames": {k: len(v) for k, v in sets.items()}, "seeds": {"1920x1080": 800000,
+                                                                                   "3840x2160": 810000},
+                          "sweep": sweep}
+    meta["threshold"] = meta["operating_points"]["fast"]["threshold"]
+    return meta
+
+
+def evaluate(api, params, meta, n=256, chunk=32, sizes=((1920, 1080), (3840, 2160)), seed0=900000, log=print):
+    """Held-out evaluation (seeds from 900000, generated chunk by chunk): per frame size,
+    the plain FP32 cascade against tile skipping at each operating point -- recall
+    (boxes paired at IoU >= 0.99, summed over the frames), tiles run, and the batch time
+    of each (DetectBatch over `chunk` frames from host memory, upload included)."""
+    from compactcnncascade_amd import recall
+    out = {}
+    for (W, H) in sizes:
+        name = "%dx%d" % (W, H)
+        dets = {}
+        for key in ("plain", "recall", "fast"):
+            d = api.FaceDetector()
+            if d.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
+                raise RuntimeError("FaceDetector.Init failed")
+            if key != "plain" and meta["operating_points"].get(key):
+                apply_operating_point(d, params, meta, key)
+            dets[key] = d
+        acc = {k: {"paired": 0, "ref": 0, "test": 0, "tiles": 0, "run": 0, "seconds": 0.0} for k in dets}
+        with ProcessPoolExecutor(max_workers=16) as ex:
+            for c0 in range(0, n, chunk):
+                frames = list(ex.map(_gen_frame, [(seed0 + c0 + i, W, H) for i in range(min(chunk, n - c0))]))
+                res = {}
+                for key, d in dets.items():
+                    d.DetectBatch(images=frames, capacity=4096)          # capture / warm-up
+                    t0 = time.perf_counter()
+                    res[key] = [f for _, f in d.DetectBatch(images=frames, capacity=4096)]
+                    acc[key]["seconds"] += time.perf_counter() - t0
+                    a, b = d.skip_stats()
+                    acc[key]["tiles"] += a
+                    acc[key]["run"] += b
+                for key in ("recall", "fast"):
+                    _, paired, nref, ntest = recall.recall(res["plain"], res[key])
+                    acc[key]["paired"] += paired
+                    acc[key]["ref"] += nref
+                    acc[key]["test"] += ntest
+                log("%s: %d/%d frames" % (name, c0 + len(frames), n))
+        row = {"frames": n, "plain_fps": round(n / acc["plain"]["seconds"], 1)}
+        for key in ("recall", "fast"):
+            a = acc[key]
+            row[key] = {"recall": round(a["paired"] / max(1, a["ref"]), 5), "paired": [a["paired"], a["ref"], a["test"]],
+                        "tiles_run_frac": round(a["run"] / max(1, a["tiles"]), 4),
+                        "fps": round(n / a["seconds"], 1),
+                        "gain_over_plain": round(acc["plain"]["seconds"] / a["seconds"], 3),
+                        "point": meta["operating_points"][key]}
+        out[name] = row
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "validate", "evaluate"])
+    ap.add_argument("--held-out", type=int, default=256, help="evaluate: frames per size")
     ap.add_argument("--frames", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--pos-weight", type=float, default=500.0)
-    ap.add_argument("--val-frames", type=int, default=24)
+    ap.add_argument("--val-fhd", type=int, default=48)
+    ap.add_argument("--val-uhd", type=int, default=12)
+    ap.add_argument("--model", default=DEFAULT_MODEL, help="model to validate")
     ap.add_argument("--out", default=DEFAULT_MODEL)
     args = ap.parse_args()
     from compactcnncascade_amd import api
-    t0 = time.time()
-    xs, ys, faces = make_dataset(api, args.frames)
-    print(json.dumps({"dataset_frames": args.frames, "faces": faces, "positive_fraction": float(ys.mean()),
-                      "seconds": round(time.time() - t0, 1)}), flush=True)
-    t1 = time.time()
-    p, losses = train(api, xs, ys, epochs=args.epochs, pos_weight=args.pos_weight)
-    train_s = round(time.time() - t1, 1)
-    del xs, ys
-    # validation frames: seeds 800000.. (training used 500000.., the tests and benchmark 900000..)
-    val = [synth.synth_frame(800000 + i, 1920, 1080, 10.0, 20, 300)[0] for i in range(args.val_frames)]
-    thr, sweep = choose_threshold(api, to_fp16(p), val)
-    meta = {"min_face": MIN_FACE, "scale_factor": SCALE_FACTOR, "slots": SLOTS, "frames": args.frames,
-            "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "pos_weight": args.pos_weight,
-            "loss": [float(v) for v in losses], "train_seconds": train_s, "threshold": thr,
-            "validation": {"frames": args.val_frames, "frame": "1920x1080", "sweep": sweep}}
+    if args.cmd == "evaluate":
+        p, meta = load(args.model)
+        print(json.dumps(evaluate(api, to_fp16(p), meta, args.held_out,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))), flush=True)
+        return
+    if args.cmd == "validate":
+        p, meta = load(args.model)
+    else:
+        t0 = time.time()
+        xs, ys, faces = make_dataset(api, args.frames)
+        print(json.dumps({"dataset_frames": args.frames, "faces": faces, "positive_fraction": float(ys.mean()),
+                          "seconds": round(time.time() - t0, 1)}), flush=True)
+        t1 = time.time()
+        p, losses = train(api, xs, ys, epochs=args.epochs, pos_weight=args.pos_weight)
+        train_s = round(time.time() - t1, 1)
+        del xs, ys
+        meta = {"min_face": MIN_FACE, "scale_factor": SCALE_FACTOR, "slots": SLOTS, "frames": args.frames,
+                "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "pos_weight": args.pos_weight,
+                "loss": [float(v) for v in losses], "train_seconds": train_s}
+    meta = validate(api, p, meta, args.val_fhd, args.val_uhd)
     save(args.out, p, meta)
-    print(json.dumps(meta), flush=True)
+    print(json.dumps(meta["operating_points"]), flush=True)
 
 
 if __name__ == "__main__":

@@ -150,6 +150,11 @@ int ccnn_autotune(ccnn_detector* d, int n, const ccnn_image* imgs, int device, i
 #define CCNN_SKIP_SLOTS 24
 #define CCNN_SKIP_PARAMS 2416
 int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int min_face, float scale_factor);
+/* Keep window of the tile mask (no counterpart in the reference): a response tile runs
+ * when some logit of the scale slots within slot_radius of its level's slot, over the
+ * cells within cell_margin of its windows' centres, reaches the threshold. Defaults 0
+ * and 1; slot_radius < CCNN_SKIP_SLOTS, cell_margin <= 16. 0, or -1. */
+int ccnn_set_skip_window(ccnn_detector* d, int slot_radius, int cell_margin);
 /* Stage-1 response tiles in the last batch and how many of them ran (read from the keep
  * flags in device memory: a synchronous copy). 0, or -1. */
 int ccnn_skip_stats(ccnn_detector* d, long long* tiles, long long* kept);

@@ -57,37 +57,67 @@ def _model():
     from compactcnncascade_amd import skipnet
     if not os.path.exists(skipnet.DEFAULT_MODEL):
         pytest.skip("no trained network committed")
-    return skipnet.load()
+    params, meta = skipnet.load()
+    assert meta["operating_points"]["recall"] and meta["operating_points"]["fast"]
+    return params, meta
 
 
 @pytest.mark.parametrize("size", [(1920, 1080, 8), (3840, 2160, 4)])
 def test_skip_ablation_on_held_out_frames(api, size):
-    # held-out seeded frames (seeds from 900000; training used seeds from 500000), faces
-    # >= 20, scale 1.15: the plain FP32 batch against the same batch with tile skipping
-    from compactcnncascade_amd import recall, synth
+    # held-out seeded frames (seeds from 900000; training used seeds from 500000,
+    # validation 800000/810000), faces >= 20, scale 1.15: the plain FP32 batch against the
+    # same batch with tile skipping at the network's two operating points
+    from compactcnncascade_amd import recall, skipnet, synth
     params, meta = _model()
     W, H, n = size
     frames = [synth.synth_frame(900000 + i, W, H, 10.0, 20, 300)[0] for i in range(n)]
-    out = {}
-    for skip in (False, True):
+    out, stats = {}, {}
+    for key in ("plain", "recall", "fast"):
         d = api.FaceDetector()
         assert d.Init(api.Param(max_image_size=(W, H), min_face_height=20, scale_factor=1.15)) == 0
+        if key != "plain":
+            skipnet.apply_operating_point(d, params, meta, key)
+        out[key] = [f for _, f in d.DetectBatch(images=frames, capacity=4096)]
+        stats[key] = d.skip_stats()
+    for key, floor in (("recall", 0.97), ("fast", 0.85)):
+        tiles, kept = stats[key]
+        rc, paired, nref, ntest = recall.recall(out["plain"], out[key])
+        print("%dx%d %s point %s: tiles run %d of %d (%.1f%%), recall %.4f (%d of %d), %d detections"
+              % (W, H, key, meta["operating_points"][key], kept, tiles, 100.0 * kept / tiles, rc, paired, nref,
+                 ntest), flush=True)
+        assert nref > 0 and kept < tiles
+        assert rc >= floor
+
+
+def test_kept_tiles_are_bit_identical_to_the_plain_cascade(api):
+    # skipping only removes work: every response of the last frame of a batch is either
+    # the plain cascade's, bit for bit, or -inf (a skipped tile)
+    from compactcnncascade_amd import skipnet, synth
+    params, meta = _model()
+    frames = [synth.synth_frame(900000 + i, 1920, 1080, 10.0, 20, 300)[0] for i in range(3)]
+    maps = {}
+    for skip in (False, True):
+        d = api.FaceDetector()
+        assert d.Init(api.Param(max_image_size=(1920, 1080), min_face_height=20, scale_factor=1.15)) == 0
         if skip:
-            d.set_skipnet(params, meta["threshold"], meta["min_face"], meta["scale_factor"])
-        out[skip] = [f for _, f in d.DetectBatch(images=frames, capacity=4096)]
-        if skip:
-            tiles, kept = d.skip_stats()
-    rc, paired, nref, ntest = recall.recall(out[False], out[True])
-    print("%dx%d: tiles run %d of %d (%.1f%%), recall %.4f (%d of %d), %d detections with skipping"
-          % (W, H, kept, tiles, 100.0 * kept / tiles, rc, paired, nref, ntest), flush=True)
-    assert nref > 0 and kept < tiles
-    assert rc >= 0.9
+            skipnet.apply_operating_point(d, params, meta, "fast")
+        d.DetectBatch(images=frames, capacity=4096)
+        maps[skip] = d.last_responses()
+    skipped = total = 0
+    for a, b in zip(maps[False], maps[True]):
+        assert a.shape == b.shape and np.isfinite(a).all()
+        off = np.isneginf(b)
+        assert np.array_equal(a[~off].view(np.uint32), b[~off].view(np.uint32))
+        skipped += int(off.sum())
+        total += a.size
+    print("responses skipped: %d of %d" % (skipped, total), flush=True)
+    assert 0 < skipped < total
 
 
 def test_skip_is_off_for_other_parameters(api):
     # a network trained for faces >= 20 at scale 1.15 does not apply to a detector with
     # faces >= 40: its batches run plain
-    from compactcnncascade_amd import synth
+    from compactcnncascade_amd import skipnet, synth
     params, meta = _model()
     frames = [synth.workload_frame("fhd_min40_max240", i)[0] for i in range(3)]
     res = []
@@ -95,7 +125,7 @@ def test_skip_is_off_for_other_parameters(api):
         d = api.FaceDetector()
         assert d.Init(api.Param(max_image_size=(1920, 1080), min_face_height=40, max_face_height=240)) == 0
         if skip:
-            d.set_skipnet(params, meta["threshold"], meta["min_face"], meta["scale_factor"])
+            skipnet.apply_operating_point(d, params, meta, "fast")
         res.append(d.DetectBatch(images=frames))
         if skip:
             tiles, kept = d.skip_stats()
