@@ -488,20 +488,29 @@ def main():
 
     # the timed region: K DetectBatch passes over the frames, for each batch mode
     modes = {}
+    step_ms = {}
+    captures = {}
     for mode in (1, 0):
         det.set_batch_mode(mode)
         for _ in range(args.warmup):
             det.DetectBatch(device_frames=dframes)
         torch.cuda.synchronize()
+        c0 = det.batch_stats()[1]
         t0 = time.perf_counter()
         faces = 0
+        marks = [t0]
         for _ in range(args.steps):
             out = det.DetectBatch(device_frames=dframes)
             faces += sum(n for n, _ in out)
+            marks.append(time.perf_counter())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         modes[mode] = (dt, faces)
-        progress("batch mode %d: %.1f frames/s" % (mode, args.steps * args.frames / dt))
+        # DetectBatch returns after the host replay, so each step's span is its own
+        step_ms[mode] = [round((b - a) * 1e3, 3) for a, b in zip(marks, marks[1:])]
+        captures[mode] = det.batch_stats()[1] - c0
+        progress("batch mode %d: %.1f frames/s, steps %s ms, graph captures %d"
+                 % (mode, args.steps * args.frames / dt, step_ms[mode], captures[mode]))
     best_mode = min(modes, key=lambda m: modes[m][0])
     dt, faces = modes[best_mode]
     nframes = args.steps * args.frames
@@ -561,6 +570,8 @@ def main():
                    "launch": ("batch graph: one captured HIP graph launch per 64-frame batch" if best_mode == 1 else
                               "per-frame launches pipelined over three streams") + ", frames resident in HBM"},
         "batch_modes_fps": {"graph": round(nframes / modes[1][0], 2), "pipelined": round(nframes / modes[0][0], 2)},
+        "step_ms": {"graph": step_ms[1], "pipelined": step_ms[0]},
+        "graph_captures_in_timed_steps": captures[1],
         "roofline": {
             "kernel": dominant,
             "bound": "valu",

@@ -101,6 +101,7 @@ def load():
     L.ccnn_set_skipnet.argtypes = [_P, _P, _F, _I, _F]
     L.ccnn_skip_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.ccnn_set_skip_window.argtypes = [_P, _I, _I]
+    L.ccnn_set_check_grid.argtypes = [_P, _I, _I]
     L.ccnn_get_compute_precision.argtypes = [_P]
     L.ccnn_stage1_ex.argtypes = [_P, _I, _I, _I, _P]
     L.ccnn_stage23_ex.argtypes = [_I, _P, _I, _I, _P]
@@ -292,6 +293,12 @@ class FaceDetector:
                                           float(np.float32(scale_factor)))
         if rc != 0:
             raise ValueError("ccnn_set_skipnet failed")
+
+    def set_check_grid(self, frame_per_cu=0, batch_per_cu=4):
+        """Grid of the check kernel: per-frame path frame_per_cu persistent workgroups per
+        CU (0: one per item), batch path batch_per_cu."""
+        if self._L.ccnn_set_check_grid(self._h, int(frame_per_cu), int(batch_per_cu)) != 0:
+            raise ValueError("bad check grid")
 
     def set_skip_window(self, slot_radius=0, cell_margin=1):
         """Keep window of the tile mask: the scale slots within slot_radius of a level's

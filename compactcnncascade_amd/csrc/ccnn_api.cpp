@@ -137,6 +137,12 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
     return d->det.set_skipnet(params, threshold, min_face, scale_factor);
 }
 
+int ccnn_set_check_grid(ccnn_detector* d, int frame_per_cu, int batch_per_cu)
+{
+    if (!d) return -1;
+    return d->det.set_check_grid(frame_per_cu, batch_per_cu);
+}
+
 int ccnn_set_skip_window(ccnn_detector* d, int slot_radius, int cell_margin)
 {
     if (!d) return -1;

@@ -299,6 +299,16 @@ public:
     // Response tiles of the last batch and how many of them ran (counted from the keep
     // flags the batch left in device memory).
     void skip_stats(long long* tiles, long long* kept);
+    // Grid of k_check: on the per-frame path frame_per_cu persistent workgroups per CU
+    // taking (candidate, crop variant) items from a queue (0: one workgroup per item,
+    // grid-stride over 1024); on the batch path batch_per_cu (1..8). 0, or -1.
+    int set_check_grid(int frame_per_cu, int batch_per_cu)
+    {
+        if (frame_per_cu < 0 || frame_per_cu > 8 || batch_per_cu < 1 || batch_per_cu > 8) return -1;
+        check_frame_per_cu_ = frame_per_cu;
+        check_per_cu_ = batch_per_cu;
+        return 0;
+    }
     // Stage-1 tile shape of the plans made from now on (see TileShape); 0 or -1 for a bad id.
     int set_tile_shape(int id)
     {
@@ -420,6 +430,7 @@ private:
     int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
     int cus_ = 256;                // compute units of the device
     int check_per_cu_ = 4;         // persistent k_check workgroups per CU on the batch path
+    int check_frame_per_cu_ = 0;   // per-frame path: 0 grid-stride over check_blocks_, else persistent
     int batch_mode_ = 1;
     long long plan_ids_ = 0;
     // stage-1 tile shape of new plans (set_tile_shape, autotune): 6 = L1 tiles of 32 x 32,

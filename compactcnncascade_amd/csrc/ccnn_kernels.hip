@@ -354,8 +354,12 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
     __shared__ float s_fx[AW], s_cx[AW], s_fy[AH], s_cy[AH];
     static_assert(AW + AH <= S1A_THREADS, "one table entry per thread");
     const int tid = threadIdx.x;
-    // the candidate count that k_stage1 accumulates starts at zero
-    if (meta && blockIdx.x == 0 && tid == 0) meta[0] = 0;
+    // the candidate count that k_stage1 accumulates starts at zero, and so does the work
+    // queue of a persistent k_check
+    if (meta && blockIdx.x == 0 && tid == 0) {
+        meta[0] = 0;
+        meta[1] = 0;
+    }
     // order (optional): the tiles as {level, tile within the level}, arranged so that the
     // contiguous run each XCD takes covers one horizontal band of every level, whose
     // gray-image rows then stay in that XCD's L2 across the levels

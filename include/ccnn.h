@@ -127,6 +127,12 @@ int ccnn_get_compute_precision(const ccnn_detector* d);
  * for a bad id. get: the current id, the shape's sizes in ax/ay/tx/ty (may be NULL). */
 #define CCNN_TILE_SHAPES 12
 int ccnn_set_tile_shape(ccnn_detector* d, int shape);
+/* Grid of the stage-2/3 check kernel (no counterpart in the reference): on the per-frame
+ * path frame_per_cu persistent workgroups per compute unit that take (candidate, crop
+ * variant) items from a queue, or 0 for one workgroup per item (grid-stride over 1024);
+ * on the batch path batch_per_cu (1..8) persistent workgroups per compute unit.
+ * Defaults 0 and 4. Results do not depend on it. 0, or -1. */
+int ccnn_set_check_grid(ccnn_detector* d, int frame_per_cu, int batch_per_cu);
 int ccnn_get_tile_shape(const ccnn_detector* d, int* ax, int* ay, int* tx, int* ty);
 /* Autotuner of the batch path: runs ccnn_detect_batch over the n frames (host images, or
  * device pointers when device != 0) reps + 1 times per stage-1 tile shape, keeps the

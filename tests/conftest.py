@@ -16,3 +16,14 @@ def pytest_configure(config):
 def oracle():
     from oracle.bind import Oracle
     return Oracle()
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_device_first(request):
+    # torch's HIP runtime must initialise before libccnn's in a process (the other order
+    # leaves torch without a device), so a run that holds GPU tests sets torch up first
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda:0")
+    yield
