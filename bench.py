@@ -125,7 +125,7 @@ def secondary_fps(api, torch, names, per_name, steps, facial=False, precisions=(
     for p in precisions:
         det = detector(api, W, H, cfg["min_face"], cfg["max_face"], cfg["scale_factor"], p, mode, facial)
         dt, res = timed_batches(torch, det, df, steps)
-        key = ["fp32", "fp16_bf16", "fp16_fp8"][p]
+        key = ["fp32", "fp16_bf16", "fp16_fp8", "fp32_bf16", "fp32_fp8"][p]
         leg = {"fps": round(len(frames) / dt, 2), "faces": int(sum(len(r) for r in res))}
         if mode == 1:
             leg["launches_per_batch"] = det.batch_stats()[0]
@@ -345,9 +345,11 @@ def pmc_summary(db, calib_db, counter, frames):
     total = 0.0
     print("# %s per dispatch; calibration: %s reads %.0f KiB for 262144 KiB moved -> factor %.4f"
           % (counter, copy[0], copy_kib, factor))
+    print("# bytes_per_frame: the detection kernels (ccnn::) only; runtime copies and fills listed apart")
     for name, (n, v) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
         b = v / n * factor * 1024
-        total += b * n
+        if "ccnn::" in name:
+            total += b * n
         print("kernel %-50s dispatches %6d avg_kib %12.1f bytes_per_dispatch %14.0f" % (name[:50], n, v / n, b))
     print("bytes_total %.0f" % total)
     print("bytes_per_frame %.0f" % (total / max(1, frames)))
@@ -607,11 +609,11 @@ def main():
         # the reduced precisions (recall against FP32); the reference's CPU-runnable case;
         # learned tile skipping; facial analysis; host frames over PCIe
         legs = [
-            ("uhd_3840x2160", lambda: secondary_fps(api, torch, ["mixed_uhd"], 8, 3, precisions=(0, 1, 2))),
+            ("uhd_3840x2160", lambda: secondary_fps(api, torch, ["mixed_uhd"], 8, 3, precisions=(0, 1, 2, 3, 4))),
             ("mixed_batch", lambda: secondary_fps(api, torch, ["mixed_vga", "mixed_hd", "mixed_fhd", "mixed_uhd"],
-                                                  8, 3, precisions=(0, 1, 2))),
+                                                  8, 3, precisions=(0, 1, 2, 3, 4))),
             ("deep_4800x2400", lambda: secondary_fps(api, torch, ["deep_4800x2400_sf105_min15"], 2, 2,
-                                                     precisions=(0, 1, 2))),
+                                                     precisions=(0, 1, 2, 3, 4))),
             ("vga_640x480", lambda: secondary_fps(api, torch, ["vga_min40"], 64, 3)),
             ("skip_fhd_min20", lambda: skip_leg(api, torch, 1920, 1080, 32, 3)),
             ("skip_uhd_min20", lambda: skip_leg(api, torch, 3840, 2160, 8, 3)),

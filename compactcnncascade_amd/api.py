@@ -22,7 +22,7 @@ PIPELINE_CPU, PIPELINE_GPU, PIPELINE_GPU_CPU = 0, 1, 2
 DETECT_DISABLE, DETECT_SYNC, DETECT_ASYNC = 0, 1, 2
 PRECISION_DEF, PRECISION_LOW, PRECISION_NORMAL, PRECISION_HIGH, PRECISION_ULTRA = 0, 1, 2, 3, 4
 # compute precisions of this library (ccnn_set_compute_precision)
-FP32, FP16_BF16, FP16_FP8 = 0, 1, 2
+FP32, FP16_BF16, FP16_FP8, FP32_BF16, FP32_FP8 = 0, 1, 2, 3, 4
 # stage-1 tile shapes by id (L1 tile, response tile), numbered as the library does:
 # id = L1 shape index * 3 + response shape index
 _L1_SHAPES = ((64, 16), (32, 16), (32, 32), (64, 8))
@@ -276,7 +276,8 @@ class FaceDetector:
             raise ValueError("bad batch mode")
 
     def set_compute_precision(self, p):
-        """FP32 (exact, default), FP16_BF16 or FP16_FP8."""
+        """FP32 (exact, default), FP16_BF16, FP16_FP8, FP32_BF16 or FP32_FP8 (stage 1, then
+        the stage-2/3 operands)."""
         if self._L.ccnn_set_compute_precision(self._h, int(p)) != 0:
             raise ValueError("bad compute precision")
 

@@ -57,6 +57,8 @@ def build(verbose=True):
            # packed FP32 instructions stay off on this target
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wno-unused-command-line-argument",
+           # every symbol the library references must resolve at link time
+           "-Wl,-z,defs",
            "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
            "-o", tmp] + srcs
     if verbose:

@@ -42,6 +42,65 @@ size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
 }  // namespace
 
+ReplayPool::ReplayPool(int threads)
+{
+    for (int t = 0; t < threads; ++t) threads_.emplace_back([this] { loop(); });
+}
+
+ReplayPool::~ReplayPool()
+{
+    {
+        std::lock_guard<std::mutex> g(m_);
+        stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto& t : threads_) t.join();
+}
+
+void ReplayPool::loop()
+{
+    long long seen = 0;
+    for (;;) {
+        const std::function<void(int)>* fn;
+        int n;
+        {
+            std::unique_lock<std::mutex> l(m_);
+            wake_.wait(l, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            // a batch that ended before this thread woke has no job left (fn_ cleared);
+            // otherwise the thread counts as busy before run() can see the batch done
+            fn = fn_;
+            n = n_;
+            if (!fn) continue;
+            busy_++;
+        }
+        for (int i; (i = next_.fetch_add(1)) < n;) (*fn)(i);
+        {
+            std::lock_guard<std::mutex> g(m_);
+            if (--busy_ == 0) done_.notify_all();
+        }
+    }
+}
+
+void ReplayPool::run(int n, const std::function<void(int)>& fn, const std::function<void()>& mine)
+{
+    {
+        std::lock_guard<std::mutex> g(m_);
+        fn_ = &fn;
+        n_ = n;
+        next_.store(0);
+        gen_++;
+    }
+    wake_.notify_all();
+    mine();
+    for (int i; (i = next_.fetch_add(1)) < n;) fn(i);
+    // every item is taken; wait for the threads still running one, then retire the job
+    std::unique_lock<std::mutex> l(m_);
+    done_.wait(l, [&] { return busy_ == 0; });
+    fn_ = nullptr;
+}
+
 void Detector::release_batch()
 {
     BatchArena& B = batch_;
@@ -328,7 +387,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
         if (e == hipSuccess && tiles > 0) {
             e = launch_stage1(B.d_gray, 0, 0, 0, B.d_levels, nl, (int)l1tiles, (int)tiles, d_w_[0], B.d_s1, B.d_resp,
                               set_.t1, nullptr, nullptr, nullptr, 0, B.d_l1_order, B.st, nullptr, seg[0].plan->shape,
-                              precision_ > 0, keep, l1_keep);
+                              half_stage1(), keep, l1_keep);
             launches += 2;
         }
         if (e == hipSuccess) {
@@ -342,7 +401,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             e = launch_check(B.d_gray, 0, 0, 0, B.d_levels, B.d_cands, B.d_meta, check_params(), d_w_[1], d_w_[2],
                              B.d_s2, B.d_s3, nullptr, check_per_cu_ * cus_, nullptr, nullptr, nullptr, B.hd_s23,
                              B.hd_s23 + 2 * B.h_cap, (int)B.h_cap, B.st, nullptr, (int)B.cand_cap, B.d_meta + 1,
-                             precision_, d_mw_[0], d_mw_[1]);
+                             check_precision(), d_mw_[0], d_mw_[1]);
             launches++;
         }
         hipGraph_t g = nullptr;
@@ -396,19 +455,14 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
                                               checks, st);
         emit_faces(rects, f.pre, faces[f.idx]);
     };
-    const int nthreads = std::max(1, std::min<int>({nf - 1, (int)std::thread::hardware_concurrency(), 8}));
-    std::atomic<int> next{0};
-    auto worker = [&]() {
-        std::vector<Candidate> lc;
-        std::vector<CheckResult> lr;
-        for (int fi; (fi = next.fetch_add(1)) < nf - 1;) run_frame(fi, lc, lr, nullptr);
+    if (!pool_) pool_.reset(new ReplayPool(std::max(0, std::min<int>((int)std::thread::hardware_concurrency(), 8) - 1)));
+    const std::function<void(int)> one = [&](int fi) {
+        thread_local std::vector<Candidate> lc;
+        thread_local std::vector<CheckResult> lr;
+        run_frame(fi, lc, lr, nullptr);
     };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker);
-    // the last frame on this thread, with its record
-    run_frame(nf - 1, last_c, last_r, &last_s);
-    worker();
-    for (auto& t : pool) t.join();
+    // the last frame on this thread, with its record; the others over the pool
+    pool_->run(nf - 1, one, [&] { run_frame(nf - 1, last_c, last_r, &last_s); });
     const BatchFrame& fl = seg[nf - 1];
     last_levels_ = fl.plan->levels;
     last_cands_.swap(last_c);

@@ -878,7 +878,7 @@ int Detector::enqueue_plan(Slot& s, const ccnn_image& img, bool device_ptr, cons
     s.host_decide = s.checks;
     HIP_OK(launch_stage1(s.d_gray, s.gpitch, gw, gh, s.d_levels, nl, plan->nl1tiles, plan->ntiles, d_w_[0], s.d_s1,
                          s.d_resp, set_.t1, s.d_meta, s.d_cands, s.host_decide ? s.hd_cands : nullptr, (int)s.pre,
-                         s.d_l1_order, st, profiling_ ? s.kev : nullptr, plan->shape, precision_ > 0));
+                         s.d_l1_order, st, profiling_ ? s.kev : nullptr, plan->shape, half_stage1()));
     if (plan->ntiles <= 0) {
         HIP_OK(hipMemsetAsync(s.d_meta, 0, 2 * sizeof(int), st));
         if (profiling_)
@@ -892,7 +892,7 @@ int Detector::enqueue_plan(Slot& s, const ccnn_image& img, bool device_ptr, cons
                             s.host_decide ? s.hd_meta : nullptr, nullptr, nullptr,
                             s.host_decide ? s.hd_s23 : nullptr, s.host_decide ? s.hd_s23 + 2 * s.h_cap : nullptr,
                             (int)s.pre, st, profiling_ ? s.kev + 4 : nullptr, 0x7fffffff,
-                            check_frame_per_cu_ > 0 ? s.d_meta + 1 : nullptr, precision_,
+                            check_frame_per_cu_ > 0 ? s.d_meta + 1 : nullptr, check_precision(),
                             d_mw_[0], d_mw_[1]));
         if (set_.facial)
             HIP_OK(launch_facial(s.d_gray, s.gpitch, gw, gh, s.d_levels, s.d_cands, s.d_meta, check_params(), s.d_s2,

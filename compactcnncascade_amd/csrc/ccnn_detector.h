@@ -6,7 +6,13 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ccnn.h"
@@ -248,6 +254,27 @@ struct BatchArena {
     int gen = 0;                  // bumped when buffers move (the graph is then recaptured)
 };
 
+// Host threads that replay the frames of a batch: started once, then woken per batch.
+// run(n, fn) calls fn(i) for every i in [0, n) on the pool's threads and on the caller,
+// whose own work `mine` runs first, and returns when all calls have returned.
+class ReplayPool {
+public:
+    explicit ReplayPool(int threads);
+    ~ReplayPool();
+    void run(int n, const std::function<void(int)>& fn, const std::function<void()>& mine);
+
+private:
+    void loop();
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable wake_, done_;
+    const std::function<void(int)>* fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int n_ = 0, busy_ = 0;
+    long long gen_ = 0;
+    bool stop_ = false;
+};
+
 class Detector {
 public:
     static constexpr int NSLOTS = 3;
@@ -275,15 +302,19 @@ public:
     void set_batch_mode(int m) { batch_mode_ = m; }
     int batch_mode() const { return batch_mode_; }
     // Compute precision: 0 the exact FP32 path (the drop-in, bit-exact with the reference);
-    // 1 FP16 stage 1 and BF16 matrix-core stages 2/3; 2 FP16 stage 1 and FP8 stages 2/3.
+    // 1 FP16 stage 1 and BF16 matrix-core stages 2/3; 2 FP16 stage 1 and FP8 stages 2/3;
+    // 3 / 4 the exact stage 1 with BF16 / FP8 stages 2/3.
     int set_precision(int p)
     {
-        if (p < 0 || p > 2) return -1;
+        if (p < 0 || p > 4) return -1;
         if (p != precision_) batch_.key.clear();
         precision_ = p;
         return 0;
     }
     int precision() const { return precision_; }
+    bool half_stage1() const { return precision_ == 1 || precision_ == 2; }
+    // operand type of the stage-2/3 kernels (RP_EXACT, RP_BF16, RP_FP8)
+    int check_precision() const { return precision_ == 0 ? 0 : (precision_ == 1 || precision_ == 3) ? 1 : 2; }
     // Tile skipping (see ccnn_set_skipnet): params NULL switches it off.
     int set_skipnet(const float* params, float threshold, int min_face, float scale_factor);
     // Keep window of the tile mask: scale slots within slot_radius of a level's own, cells
@@ -439,6 +470,7 @@ private:
     int shape_ = 6;
     int batch_captures_ = 0;
     BatchArena batch_;
+    std::unique_ptr<ReplayPool> pool_;   // host replay threads of the batch path
     int fa_blocks_ = 64;           // k_facial workgroups (76 KB LDS each); windows dealt round-robin
 };
 

@@ -998,9 +998,9 @@ static_assert(12 * chk::P2N <= chk::IH * chk::IW && 24 * chk::RN + chk::RN <= ch
 // 2x2 max-pool is two lane swaps. The weights matrices are expanded per lane by the host
 // (MfmaW) and rounded to the operand type in the kernel. FP8 (e4m3) operands are scaled
 // by powers of two: the weights per layer (host), the L2 input per window (its largest
-// magnitude), and the products are scaled back exactly; activations and weights enter as
-// two FP8 terms each (value and rounding error), three products per pair. L3, the hidden
-// and output layers run in FP32 as on the exact path.
+// magnitude), and the products are scaled back exactly. Activations and weights enter as
+// two terms of the operand type each (value and rounding error), three products per
+// pair. L3, the hidden and output layers run in FP32 as on the exact path.
 // ---------------------------------------------------------------------------------
 enum { RP_EXACT = 0, RP_BF16 = 1, RP_FP8 = 2 };
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -1017,6 +1017,12 @@ struct MOp<RP_BF16> {
 #pragma unroll
         for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
         return r;
+    }
+    // the BF16 values of an operand, back in FP32 (exact)
+    static __device__ __forceinline__ void unpack(T t, float (&v)[8])
+    {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (float)t[i];
     }
     static __device__ __forceinline__ f32x4_t mma(T a, T b, f32x4_t c)
     {
@@ -1049,47 +1055,42 @@ struct MOp<RP_FP8> {
     }
 };
 
-// The weight operand of one lane: BF16 one term; FP8 two terms, the rounded value and its
-// rounding error, so that weight and activation each carry about seven significant bits.
+// Operands enter as two terms of the operand type each, the rounded value and its
+// rounding error (BF16: about 16 significant bits together, FP8 e4m3 about 8), and each
+// product pair as three MFMAs into one FP32 accumulator: hi x hi, lo x hi, hi x lo (lo x lo
+// lies below the accumulator's rounding).
 template <int RP>
 struct WOp {
     typename MOp<RP>::T hi, lo;
 };
 template <int RP>
-__device__ __forceinline__ WOp<RP> pack_w(const float (&w)[8])
+__device__ __forceinline__ typename MOp<RP>::T rest_of(const float (&w)[8], typename MOp<RP>::T hi)
 {
     using Op = MOp<RP>;
-    WOp<RP> r;
-    r.hi = Op::pack(w);
-    if constexpr (RP == RP_FP8) {
-        float v[8];
-        Op::unpack(r.hi, v);
+    float v[8];
+    Op::unpack(hi, v);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = w[j] - v[j];
-        r.lo = Op::pack(v);
-    } else {
-        r.lo = r.hi;
-    }
+    for (int j = 0; j < 8; ++j) v[j] = w[j] - v[j];
+    return Op::pack(v);
+}
+template <int RP>
+__device__ __forceinline__ WOp<RP> pack_w(const float (&w)[8])
+{
+    WOp<RP> r;
+    r.hi = MOp<RP>::pack(w);
+    r.lo = rest_of<RP>(w, r.hi);
     return r;
 }
 
-// The activation operand times the weights. FP8: the activations as the sum of two FP8
-// terms too (exact for the 8-bit pixels of L1), three products into one accumulator
-// (hi x hi, lo x hi, hi x lo; lo x lo is below the accumulator's rounding); BF16: one.
+// The activation operand (eight values of one lane) times the weights.
 template <int RP>
 __device__ __forceinline__ f32x4_t mma_act(const float (&a)[8], const WOp<RP>& b, f32x4_t acc)
 {
     using Op = MOp<RP>;
     const typename Op::T ah = Op::pack(a);
     acc = Op::mma(ah, b.hi, acc);
-    if constexpr (RP == RP_FP8) {
-        float r[8];
-        Op::unpack(ah, r);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) r[j] = a[j] - r[j];
-        acc = Op::mma(Op::pack(r), b.hi, acc);
-        acc = Op::mma(ah, b.lo, acc);
-    }
+    acc = Op::mma(rest_of<RP>(a, ah), b.hi, acc);
+    acc = Op::mma(ah, b.lo, acc);
     return acc;
 }
 

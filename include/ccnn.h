@@ -112,10 +112,13 @@ int ccnn_set_batch_mode(ccnn_detector* d, int mode);
  *                             layers as BF16 GEMMs on the matrix cores with FP32
  *                             accumulation;
  *   CCNN_PRECISION_FP16_FP8   the same with FP8 (e4m3) operands for stages 2/3, scaled
- *                             per layer by powers of two.
+ *                             per layer by powers of two;
+ *   CCNN_PRECISION_FP32_BF16  the exact FP32 stage 1 with the BF16 stages 2/3;
+ *   CCNN_PRECISION_FP32_FP8   the exact FP32 stage 1 with the FP8 stages 2/3.
  * Reduced precisions change scores and may change the detection set; recall against the
  * FP32 path is stated by the project's tests and benchmark. 0, or -1 for a bad value. */
-enum { CCNN_PRECISION_FP32 = 0, CCNN_PRECISION_FP16_BF16 = 1, CCNN_PRECISION_FP16_FP8 = 2 };
+enum { CCNN_PRECISION_FP32 = 0, CCNN_PRECISION_FP16_BF16 = 1, CCNN_PRECISION_FP16_FP8 = 2,
+       CCNN_PRECISION_FP32_BF16 = 3, CCNN_PRECISION_FP32_FP8 = 4 };
 int ccnn_set_compute_precision(ccnn_detector* d, int precision);
 int ccnn_get_compute_precision(const ccnn_detector* d);
 

@@ -72,7 +72,7 @@ def test_stage23_bf16_fp8_error_vs_fp64(api, oracle):
     xs3 += list(np.round(rng.random((8, 51, 47)) * 255).astype(np.float32))
     # responses lie in [-1.7159, 1.7159]; BF16 / FP8 L1 and L2 move the 4x4 maps of real
     # windows by up to about 0.12 / 0.3 (measured), well inside the threshold margins
-    bounds = {api.FP32: 2e-5, api.FP16_BF16: 0.3, api.FP16_FP8: 0.3}
+    bounds = {api.FP32: 2e-5, api.FP16_BF16: 1e-3, api.FP16_FP8: 0.3}
     for stage, xs in ((1, np.stack(xs2)), (2, np.stack(xs3))):
         e64 = np.stack([forward64(oracle.models[stage], x) for x in xs])
         for prec, bound in bounds.items():
@@ -94,18 +94,17 @@ def test_reduced_precision_recall_against_fp32(api, which):
         frames = [synth.workload_frame("deep_4800x2400_sf105_min15", 0)[0]]
         W, H, mf, sf = 4800, 2400, 15, 1.05
     res = {}
-    for prec in (api.FP32, api.FP16_BF16, api.FP16_FP8):
+    for prec in (api.FP32, api.FP16_BF16, api.FP16_FP8, api.FP32_BF16, api.FP32_FP8):
         d = api.FaceDetector()
         assert d.Init(api.Param(max_image_size=(W, H), min_face_height=mf, scale_factor=sf)) == 0
         d.set_compute_precision(prec)
         res[prec] = [f for _, f in d.DetectBatch(images=frames, capacity=20000)]
-    floors = {api.FP16_BF16: 0.9, api.FP16_FP8: 0.5}
+    # a reduced precision moves scores near the thresholds, and a window that crosses one
+    # changes which boxes Merger averages: the floors sit below the measured recall
+    floors = {api.FP16_BF16: 0.9, api.FP16_FP8: 0.9, api.FP32_BF16: 0.9, api.FP32_FP8: 0.9}
     for prec, floor in floors.items():
         rc, paired, nref, ntest = recall.recall(res[api.FP32], res[prec])
         print("%s precision %d: recall %.4f (%d of %d FP32 detections paired at IoU >= 0.99; %d detections)"
               % (which, prec, rc, paired, nref, ntest), flush=True)
         assert nref > 10
-        # a reduced precision moves scores near the thresholds, and a window that crosses
-        # one changes which boxes Merger averages: the floors are the measured recall
-        # rounded down (BF16 about 0.96, FP8 about 0.7 before the two-term activations)
         assert rc >= floor
