@@ -159,7 +159,7 @@ int Detector::detect_batch_graph(int n, const ccnn_image* imgs, bool device_ptr,
         f.h = im.rows;
         f.ch = im.channels;
         f.step = im.step;
-        f.plan = plan_frame(im, &f.pre, &f.gw, &f.gh);
+        f.plan = plan_frame(im, &f.pre, &f.gw, &f.gh, shape_);
         const_cast<Plan*>(f.plan)->pins++;
         seg.push_back(f);
     }

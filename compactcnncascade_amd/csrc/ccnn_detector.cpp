@@ -607,10 +607,9 @@ int Detector::clear()
 // Level table for a gray image of gw x gh: levels scl_max -> 0 (RunCPUDetect order,
 // which is also the RunCheckDetectAsync order), skip rule (:2083-2088), resize mode
 // (:2093-2104), response size (getOutputImgSize, cnn_simd_cntk.cpp:997-1012).
-const Plan* Detector::plan_for(int gw, int gh, int ncap)
+const Plan* Detector::plan_for(int gw, int gh, int ncap, int shape)
 {
     ncap = std::min(std::max(ncap, 0), (int)scales_.size());
-    const int shape = shape_;
     for (const Plan* p : plans_)
         if (p->gw == gw && p->gh == gh && p->ncap == ncap && p->shape == shape) return p;
     Plan* P = new Plan();
@@ -816,13 +815,13 @@ CheckParams Detector::check_params() const
 // Launches one frame on slot s: upload (host frames), optional 8-bit pre-resize, gray,
 // stage 1 over all levels (which appends the candidate windows), stage-2/3 checks,
 // asynchronous read-back.
-const Plan* Detector::plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh)
+const Plan* Detector::plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh, int shape)
 {
     *gw = img.cols;
     *gh = img.rows;
     *pre = gray_only_ && pre_scale_ > 0.f && img.channels == 1;
     if (*pre) size_mul(img.cols, img.rows, pre_scale_, gw, gh);
-    const Plan* plan = plan_for(*gw, *gh, num_scales_);
+    const Plan* plan = plan_for(*gw, *gh, num_scales_, shape);
     num_scales_ = plan->next_num_scales;
     return plan;
 }
@@ -831,7 +830,7 @@ int Detector::enqueue(Slot& s, const ccnn_image& img, bool device_ptr)
 {
     bool pre;
     int gw, gh;
-    const Plan* plan = plan_frame(img, &pre, &gw, &gh);
+    const Plan* plan = plan_frame(img, &pre, &gw, &gh, frame_shape_);
     return enqueue_plan(s, img, device_ptr, plan, pre, gw, gh);
 }
 

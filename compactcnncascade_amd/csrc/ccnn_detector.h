@@ -344,7 +344,7 @@ public:
     int set_tile_shape(int id)
     {
         if (id < 0 || id >= N_TILE_SHAPES) return -1;
-        shape_ = id;
+        shape_ = frame_shape_ = id;
         return 0;
     }
     int tile_shape_id() const { return shape_; }
@@ -390,13 +390,13 @@ private:
     int init_device();
     int build();                                  // Init chain after parameter changes
     int prepare(const ccnn_image& img);           // size guards and re-initialisation of Detect
-    const Plan* plan_for(int gw, int gh, int ncap);
+    const Plan* plan_for(int gw, int gh, int ncap, int shape);
     int ensure_slot(Slot& s, int w, int h, size_t img_bytes, const Plan& plan);
     int enqueue(Slot& s, const ccnn_image& img, bool device_ptr);
     // launches one planned frame on slot s
     int enqueue_plan(Slot& s, const ccnn_image& img, bool device_ptr, const Plan* plan, bool pre, int gw, int gh);
     // the frame's pre-resize, gray size and plan; advances num_scales_
-    const Plan* plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh);
+    const Plan* plan_frame(const ccnn_image& img, bool* pre, int* gw, int* gh, int shape);
     int detect_batch_pipelined(int n, const ccnn_image* imgs, bool device_ptr,
                                std::vector<std::vector<ccnn_face>>& faces);
     int detect_batch_graph(int n, const ccnn_image* imgs, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces);
@@ -464,10 +464,13 @@ private:
     int check_frame_per_cu_ = 0;   // per-frame path: 0 grid-stride over check_blocks_, else persistent
     int batch_mode_ = 1;
     long long plan_ids_ = 0;
-    // stage-1 tile shape of new plans (set_tile_shape, autotune): 6 = L1 tiles of 32 x 32,
-    // response tiles of 16 x 16, the autotuner's pick on gfx950 for the 1080p, 4K and
-    // mixed-size batches (bench.py --autotune)
+    // stage-1 tile shape of new plans of the batch path (set_tile_shape, autotune): 6 = L1
+    // tiles of 32 x 32, response tiles of 16 x 16, the autotuner's pick on gfx950 for the
+    // 1080p, 4K and mixed-size batches (bench.py --autotune); and of the per-frame path
+    // (set_tile_shape): 0 = L1 tiles of 64 x 16, 2% faster than 6 for one 1080p frame
+    // alone on the GPU
     int shape_ = 6;
+    int frame_shape_ = 0;
     int batch_captures_ = 0;
     BatchArena batch_;
     std::unique_ptr<ReplayPool> pool_;   // host replay threads of the batch path

@@ -125,9 +125,11 @@ int ccnn_get_compute_precision(const ccnn_detector* d);
 /* Stage-1 tile shapes (no counterpart in the reference). The two stage-1 kernels are
  * built for CCNN_TILE_SHAPES combinations of an L1 tile (pooled L1 positions per
  * workgroup: 64x16, 32x16, 32x32, 64x8) and a response tile (responses per workgroup:
- * 16x16, 32x8, 16x8); shape id = L1 index * 3 + response index, 0 the default. The
- * results do not depend on the shape. set: for the frames planned from now on; 0, or -1
- * for a bad id. get: the current id, the shape's sizes in ax/ay/tx/ty (may be NULL). */
+ * 16x16, 32x8, 16x8); shape id = L1 index * 3 + response index. Defaults: 6 (L1 32x32,
+ * responses 16x16, the autotuner's pick on gfx950) on the batch path, 0 (L1 64x16,
+ * responses 16x16) on the per-frame path. The results do not depend on the shape. set:
+ * both paths, for the frames planned from now on; 0, or -1 for a bad id. get: the batch
+ * path's id (the one ccnn_autotune sets), its sizes in ax/ay/tx/ty (may be NULL). */
 #define CCNN_TILE_SHAPES 12
 int ccnn_set_tile_shape(ccnn_detector* d, int shape);
 /* Grid of the stage-2/3 check kernel (no counterpart in the reference): on the per-frame
