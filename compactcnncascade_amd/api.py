@@ -111,6 +111,7 @@ def load():
     L.ccnn_set_profiling.argtypes = [_P, _I]
     L.ccnn_last_error.argtypes = [_I]
     L.ccnn_last_timing.argtypes = [_P, _P]
+    L.ccnn_last_batch_timing.argtypes = [_P, _P]
     _lib = L
     return L
 
@@ -422,6 +423,17 @@ class FaceDetector:
         last = {k: float(ms[i]) for i, k in enumerate(keys)}
         totals = {k: float(ms[7 + i]) for i, k in enumerate(keys)}
         return dict(last=last, totals=totals, frames=int(ms[14]))
+
+    def last_batch_timing(self):
+        """Milliseconds per kernel group of the last batch run with profiling on (the
+        kernels then run without the graph): gray, skipnet, stage1_l1, stage1, compact,
+        check, total, and the batch's frame count."""
+        ms = np.zeros(8, np.float32)
+        self._L.ccnn_last_batch_timing(self._h, _ptr(ms))
+        keys = ("gray", "skipnet", "stage1_l1", "stage1", "compact", "check", "total")
+        out = {k: float(ms[i]) for i, k in enumerate(keys)}
+        out["frames"] = int(ms[7])
+        return out
 
     def last_facial(self):
         """Facial analysis of the last frame's candidates: int array [n, 4] of

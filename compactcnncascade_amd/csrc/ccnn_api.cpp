@@ -137,6 +137,13 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
     return d->det.set_skipnet(params, threshold, min_face, scale_factor);
 }
 
+int ccnn_last_batch_timing(const ccnn_detector* d, float* ms)
+{
+    if (!d || !ms) return -1;
+    d->det.batch_timing(ms);
+    return 0;
+}
+
 int ccnn_set_check_grid(ccnn_detector* d, int frame_per_cu, int batch_per_cu)
 {
     if (!d) return -1;

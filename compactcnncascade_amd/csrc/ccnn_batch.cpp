@@ -111,6 +111,8 @@ void Detector::release_batch()
     dfree(B.d_l1_order); dfree(B.d_ftab); dfree(B.d_counts); dfree(B.d_offs); dfree(B.d_meta); dfree(B.d_cands);
     dfree(B.d_s2); dfree(B.d_s3); dfree(B.d_snf); dfree(B.d_sn); dfree(B.d_keep);
     hfree(B.h_ftab); hfree(B.h_meta); hfree(B.h_cands); hfree(B.h_s23);
+    for (auto& ev : B.ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (B.st) { (void)hipStreamDestroy(B.st); B.st = nullptr; }
     const int gen = B.gen + 1;
     B = BatchArena();
@@ -346,20 +348,16 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             HIP_OK(hipMemcpyAsync(B.d_frames + foff[fi], f.data, bytes, hipMemcpyHostToDevice, B.st));
         }
 
-    if (key != B.key || !B.exec) {
-        // the level table and tile order of this layout, then the graph
-        HIP_OK(hipMemcpyAsync(B.d_levels, lv.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, B.st));
-        if (!order.empty())
-            HIP_OK(hipMemcpyAsync(B.d_l1_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
-                                  B.st));
-        HIP_OK(hipMemcpyAsync(B.d_snf, snf.data(), sizeof(SnFrame) * nf, hipMemcpyHostToDevice, B.st));
-        HIP_OK(hipStreamSynchronize(B.st));
-        if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
-        if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
-        B.key.clear();
-        HIP_OK(hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal));
+    // the launch sequence of the batch; ev (optional): events recorded after each group of
+    // kernels (gray, proposal network, L1 sums, stage-1 tiles, compaction, checks), ev[0]
+    // before the first
+    auto launch_all = [&](hipEvent_t* ev, int* nlaunch) -> hipError_t {
         int launches = 0;
         hipError_t e = hipSuccess;
+        auto mark = [&](int k) {
+            if (ev && e == hipSuccess) e = hipEventRecord(ev[k], B.st);
+        };
+        mark(0);
         for (int fi = 0; fi < nf && e == hipSuccess; ++fi)
             if (seg[fi].pre) {
                 e = launch_resize_u8_tab(B.d_ftab, fi, seg[fi].gw, seg[fi].gh, B.st);
@@ -369,6 +367,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             e = launch_gray_batch(B.d_ftab, nf, (int)quads, B.d_gray, B.st);
             launches++;
         }
+        mark(1);
         unsigned char* keep = nullptr;
         unsigned char* l1_keep = nullptr;
         if (e == hipSuccess && skip && tiles > 0) {
@@ -384,40 +383,89 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
                                skip_margin_, keep, l1_keep, B.st);
             launches += 6;
         }
+        mark(2);
         if (e == hipSuccess && tiles > 0) {
+            // with events, the two stage-1 kernels are timed apart (kev: start/stop pairs)
             e = launch_stage1(B.d_gray, 0, 0, 0, B.d_levels, nl, (int)l1tiles, (int)tiles, d_w_[0], B.d_s1, B.d_resp,
-                              set_.t1, nullptr, nullptr, nullptr, 0, B.d_l1_order, B.st, nullptr, seg[0].plan->shape,
-                              half_stage1(), keep, l1_keep);
+                              set_.t1, nullptr, nullptr, nullptr, 0, B.d_l1_order, B.st, ev ? ev + 7 : nullptr,
+                              seg[0].plan->shape, half_stage1(), keep, l1_keep);
             launches += 2;
+        } else if (ev) {
+            for (int k = 7; k < 11 && e == hipSuccess; ++k) e = hipEventRecord(ev[k], B.st);
         }
+        mark(3);
         if (e == hipSuccess) {
             e = launch_compact(B.d_resp, (int)resp_n, set_.t1, B.d_counts, B.d_offs, B.d_levels, nl, B.d_cands,
                                (int)B.cand_cap, B.hd_cands, (int)B.h_cap, B.d_meta, B.hd_meta, B.st);
             launches += nb > 0 ? 3 : 1;
         }
+        mark(4);
         if (e == hipSuccess && checks) {
-            // a persistent grid of four workgroups per CU takes the items from the queue
-            // (meta[1], zeroed by the compaction)
+            // a persistent grid of check_per_cu workgroups per CU takes the items from the
+            // queue (meta[1], zeroed by the compaction)
             e = launch_check(B.d_gray, 0, 0, 0, B.d_levels, B.d_cands, B.d_meta, check_params(), d_w_[1], d_w_[2],
                              B.d_s2, B.d_s3, nullptr, check_per_cu_ * cus_, nullptr, nullptr, nullptr, B.hd_s23,
                              B.hd_s23 + 2 * B.h_cap, (int)B.h_cap, B.st, nullptr, (int)B.cand_cap, B.d_meta + 1,
                              check_precision(), d_mw_[0], d_mw_[1]);
             launches++;
         }
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(B.st, &g);
-        if (e != hipSuccess || ec != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            HIP_OK(e != hipSuccess ? e : ec);
-        }
-        B.graph = g;
-        HIP_OK(hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0));
-        B.key = key;
-        B.launches = launches;
-        batch_captures_++;
+        mark(5);
+        if (nlaunch) *nlaunch = launches;
+        return e;
+    };
+    const bool fresh = key != B.key;
+    if (fresh) {
+        // the level table and tile order of this layout
+        HIP_OK(hipMemcpyAsync(B.d_levels, lv.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, B.st));
+        if (!order.empty())
+            HIP_OK(hipMemcpyAsync(B.d_l1_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
+                                  B.st));
+        HIP_OK(hipMemcpyAsync(B.d_snf, snf.data(), sizeof(SnFrame) * nf, hipMemcpyHostToDevice, B.st));
+        HIP_OK(hipStreamSynchronize(B.st));
+        if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
+        if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
+        B.key.clear();
     }
-    HIP_OK(hipGraphLaunch(B.exec, B.st));
-    HIP_OK(hipStreamSynchronize(B.st));
+    if (profiling_) {
+        // profiled batches run the same kernels without the graph, with events between the
+        // groups, so that each group's time is its own (the tables above are uploaded for
+        // this layout; the graph, if any, stays as it is)
+        if (!B.ev[0])
+            for (auto& ev : B.ev) HIP_OK(hipEventCreate(&ev));
+        int launches = 0;
+        HIP_OK(launch_all(B.ev, &launches));
+        HIP_OK(hipStreamSynchronize(B.st));
+        const int pairs[6][2] = {{0, 1}, {1, 2}, {7, 8}, {9, 10}, {3, 4}, {4, 5}};
+        for (int k = 0; k < 6; ++k) {
+            float t = 0.f;
+            HIP_OK(hipEventElapsedTime(&t, B.ev[pairs[k][0]], B.ev[pairs[k][1]]));
+            B.ms[k] = t;
+        }
+        float t = 0.f;
+        HIP_OK(hipEventElapsedTime(&t, B.ev[0], B.ev[5]));
+        B.ms[6] = t;
+        B.ms[7] = (float)nf;
+        B.launches = launches;
+    } else {
+        if (fresh || !B.exec) {
+            HIP_OK(hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal));
+            int launches = 0;
+            const hipError_t e = launch_all(nullptr, &launches);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(B.st, &g);
+            if (e != hipSuccess || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HIP_OK(e != hipSuccess ? e : ec);
+            }
+            B.graph = g;
+            HIP_OK(hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0));
+            B.key = key;
+            B.launches = launches;
+            batch_captures_++;
+        }
+        HIP_OK(hipGraphLaunch(B.exec, B.st));
+        HIP_OK(hipStreamSynchronize(B.st));
+    }
 
     B.tiles = tiles;
     B.skipped = skip && tiles > 0;

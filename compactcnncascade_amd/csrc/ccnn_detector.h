@@ -250,6 +250,11 @@ struct BatchArena {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<long long> key;   // what the captured graph was built for
+    // profiled batches: events between the kernel groups, and the last batch's ms per
+    // group (gray, proposal network, L1 sums, stage-1 tiles, compaction, checks), the
+    // whole sequence, and its frame count
+    hipEvent_t ev[11] = {};
+    float ms[8] = {};
     int launches = 0;             // kernel launches in the graph
     int gen = 0;                  // bumped when buffers move (the graph is then recaptured)
 };
@@ -353,6 +358,11 @@ public:
     // returns its shape id; ms[s] receives each shape's best batch time (N_TILE_SHAPES
     // entries, NaN for shapes that failed) and grid_ms[g] each grid's (3 entries).
     int autotune(int n, const ccnn_image* imgs, bool device_ptr, int reps, float* ms, float* grid_ms);
+    // ms per kernel group of the last profiled batch (see BatchArena::ms): 8 floats
+    void batch_timing(float* ms) const
+    {
+        for (int k = 0; k < 8; ++k) ms[k] = batch_.ms[k];
+    }
     // kernel launches of the last batch graph, graph captures so far
     int batch_launches() const { return batch_.launches; }
     int batch_captures() const { return batch_captures_; }
@@ -460,7 +470,7 @@ private:
     int acc_frames_ = 0;
     int check_blocks_ = 1024;      // (candidate, crop variant) workgroups of k_check, grid-stride
     int cus_ = 256;                // compute units of the device
-    int check_per_cu_ = 4;         // persistent k_check workgroups per CU on the batch path
+    int check_per_cu_ = 1024 / CHECK_THREADS;   // persistent k_check workgroups per CU on the batch path
     int check_frame_per_cu_ = 0;   // per-frame path: 0 grid-stride over check_blocks_, else persistent
     int batch_mode_ = 1;
     long long plan_ids_ = 0;

@@ -321,6 +321,12 @@ int ccnn_last_error(int reset);
  * for the last frame; ms[7..13] the same summed over the frames since profiling was
  * enabled; ms[14] that frame count. ms must hold 15 floats. Returns 0. */
 void ccnn_set_profiling(ccnn_detector* d, int on);
+/* Batch path with profiling on (no counterpart in the reference): each batch runs its
+ * kernels without the graph, with HIP events between the kernel groups. ms (8 floats)
+ * receives the last such batch's milliseconds for: gray (and the 8-bit pre-resizes), the
+ * proposal network and tile masks, k_stage1_l1, k_stage1, the compaction, k_check, the
+ * whole sequence, and the batch's frame count. 0, or -1. */
+int ccnn_last_batch_timing(const ccnn_detector* d, float* ms);
 int ccnn_last_timing(const ccnn_detector* d, float* ms);
 
 #ifdef __cplusplus
