@@ -42,8 +42,11 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 FP8_PEAK_TFLOPS = 5000.0
 # The exact path evaluates every multiply and add as its own rounded VALU op (no FMA,
-# no packed FP32): 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3 T ops/s.
-VALU_EXACT_PEAK_TOPS = 39.3
+# no packed FP32), so it reaches at most half the FP32 peak (which counts an FMA as two
+# FLOP): 256 CUs x 4 SIMD x 32 FP32 lanes per clock x 2.4 GHz = 78.6 T ops/s. (The SQ
+# counters confirm the 32 lanes per clock: in a 1080p batch k_stage1_l1 issues 0.32
+# wave64 VALU instructions per SIMD per clock, above the 0.25 that 16 lanes would allow.)
+VALU_EXACT_PEAK_TOPS = 78.6
 STAGE2_MAC = 278496
 STAGE3_MAC = 276672
 

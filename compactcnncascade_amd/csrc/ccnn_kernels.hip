@@ -673,17 +673,22 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         }
     }
     __syncthreads();
-    float* s_s2 = s_s1;   // L2 sums overwrite the (consumed) L1 sums
+    // L2 sums overwrite the (consumed) L1 sums, with rows P2S floats apart: an odd multiple
+    // of 8, so that the rows of an 8 x 8 block of responses (below) start in distinct
+    // groups of 8 LDS banks and the L3 window reads of a wave are free of bank conflicts
+    constexpr int P2S = (P2W + 7) / 16 * 16 + 8, P2NS = P2S * G::P2H;
+    static_assert(P2S >= P2W && N::M2 * P2NS <= N::M1 * P1N, "L2 sums fit in the L1-sum space");
+    float* s_s2 = s_s1;
     for (int r = tid; r < nq; r += S1_THREADS) {
         const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
-        const int p = qy * P2W + qx;
+        const int p = qy * P2W + qx, ps = qy * P2S + qx;
         float v[N::M2];
 #pragma unroll
         for (int m = 0; m < N::M2; ++m) v[m] = s_p2[m * P2N + p];
-        s_s2[p] = v[0] + v[1];
+        s_s2[ps] = v[0] + v[1];
 #pragma unroll
-        for (int i = 1; i < N::M2 - 1; ++i) s_s2[i * P2N + p] = v[i - 1] + v[i] + v[i + 1];
-        s_s2[(N::M2 - 1) * P2N + p] = v[N::M2 - 2] + v[N::M2 - 1];
+        for (int i = 1; i < N::M2 - 1; ++i) s_s2[i * P2NS + ps] = v[i - 1] + v[i] + v[i + 1];
+        s_s2[(N::M2 - 1) * P2NS + ps] = v[N::M2 - 2] + v[N::M2 - 1];
     }
     __syncthreads();
     float* s_wt = s_p2;
@@ -700,22 +705,34 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     __syncthreads();
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
-    // then the hidden and output layers in registers, one response value per thread.
-    // threads past the responses inside the map stop here (no barrier follows)
-    if (tid >= vw * vh) return;
-    const int oy = (int)(((float)tid + 0.5f) * rvw), ox = tid - oy * vw;
+    // then the hidden and output layers in registers, one response value per thread. In a
+    // whole tile each wave takes an 8 x 8 block of responses; in an edge tile the responses
+    // inside the map are numbered densely and the threads past them stop here (no barrier
+    // follows)
+    static_assert(TX % 8 == 0 && TY % 8 == 0 && TX * TY % 64 == 0, "8 x 8 blocks of responses");
+    int oy, ox;
+    if (vw == TX && vh == TY) {
+        if (tid >= TX * TY) return;
+        const int b = tid >> 6, l = tid & 63;
+        oy = (b / (TX / 8)) * 8 + (l >> 3);
+        ox = (b % (TX / 8)) * 8 + (l & 7);
+    } else {
+        if (tid >= vw * vh) return;
+        oy = (int)(((float)tid + 0.5f) * rvw);
+        ox = tid - oy * vw;
+    }
     const float* Wt = s_wt - N::oK3;   // indexed with the offsets into W
     const float* k3 = Wt + N::oK3;
     const float* b3 = Wt + N::oB3;
     float a3[N::M3];
 #pragma unroll
     for (int i = 0; i < N::M2; ++i) {
-        const float* src = s_s2 + i * P2N + oy * P2W + ox;
+        const float* src = s_s2 + i * P2NS + oy * P2S + ox;
         float win[N::K3H][N::K3W];
 #pragma unroll
         for (int r = 0; r < N::K3H; ++r)
 #pragma unroll
-            for (int c = 0; c < N::K3W; ++c) win[r][c] = src[r * P2W + c];
+            for (int c = 0; c < N::K3W; ++c) win[r][c] = src[r * P2S + c];
         if constexpr (H) {
             static_assert(N::K3W == 4, "two tap pairs per L3 row");
             h2_t hp[N::K3H][2];
