@@ -101,6 +101,7 @@ def load():
     L.ccnn_set_skipnet.argtypes = [_P, _P, _F, _I, _F]
     L.ccnn_skip_stats.argtypes = [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L.ccnn_set_skip_window.argtypes = [_P, _I, _I]
+    L.ccnn_set_skip_slot_thresholds.argtypes = [_P, _P]
     L.ccnn_set_check_grid.argtypes = [_P, _I, _I]
     L.ccnn_get_compute_precision.argtypes = [_P]
     L.ccnn_stage1_ex.argtypes = [_P, _I, _I, _I, _P]
@@ -307,6 +308,12 @@ class FaceDetector:
         own and the cells within cell_margin of its windows' centres."""
         if self._L.ccnn_set_skip_window(self._h, int(slot_radius), int(cell_margin)) != 0:
             raise ValueError("bad skip window")
+
+    def set_skip_slot_thresholds(self, thresholds):
+        """Keep threshold per scale slot (24 probabilities in (0, 1))."""
+        self._skip_thr = np.ascontiguousarray(thresholds, np.float32)
+        if self._skip_thr.size != 24 or self._L.ccnn_set_skip_slot_thresholds(self._h, _ptr(self._skip_thr)) != 0:
+            raise ValueError("bad slot thresholds")
 
     def skip_stats(self):
         """(stage-1 response tiles of the last batch, tiles run)"""

@@ -150,6 +150,12 @@ int ccnn_set_check_grid(ccnn_detector* d, int frame_per_cu, int batch_per_cu)
     return d->det.set_check_grid(frame_per_cu, batch_per_cu);
 }
 
+int ccnn_set_skip_slot_thresholds(ccnn_detector* d, const float* thresholds)
+{
+    if (!d || !thresholds) return -1;
+    return d->det.set_skip_slot_thresholds(thresholds);
+}
+
 int ccnn_set_skip_window(ccnn_detector* d, int slot_radius, int cell_margin)
 {
     if (!d) return -1;

@@ -189,7 +189,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     // tile skipping applies to the parameters the network was trained for
     const bool skip = d_skip_ && param.min_face_height == skip_min_face_ && param.scale_factor == skip_sf_;
     key.push_back(skip);
-    key.push_back(skip ? (long long)(skip_threshold_ * 1e6f) : 0);
+    for (int k = 0; k < CCNN_SKIP_SLOTS; ++k) key.push_back(skip ? (long long)(skip_threshold_[k] * 1e6f) : 0);
     key.push_back(skip ? skip_srad_ * 64 + skip_margin_ : 0);
     for (const BatchFrame& f : seg)
         if (f.plan->shape != seg[0].plan->shape) return 1;   // one tile shape per launch
@@ -549,7 +549,7 @@ int Detector::set_skipnet(const float* params, float threshold, int min_face, fl
     for (float& v : w) v = (float)(_Float16)v;
     if (dalloc(d_skip_, w.size()) < 0) return -1;
     HIP_OK(hipMemcpy(d_skip_, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
-    skip_threshold_ = threshold;
+    for (float& t : skip_threshold_) t = threshold;
     skip_min_face_ = min_face;
     skip_sf_ = scale_factor;
     return 0;

@@ -47,7 +47,7 @@ hipError_t launch_resize_u8_tab(const FrameDesc* frames, int f, int dw, int dh, 
 // response tiles (keep) and L1 tiles (l1_keep); kept (optional) counts the kept tiles
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
-                          int ntiles, int nl1tiles, int shape, float threshold, int srad, int margin,
+                          int ntiles, int nl1tiles, int shape, const float* threshold, int srad, int margin,
                           unsigned char* keep, unsigned char* l1_keep, hipStream_t st);
 // gray of every frame of a batch (FrameDesc table in device memory)
 hipError_t launch_gray_batch(const FrameDesc* frames, int nframes, int total_quads, float* gray, hipStream_t st);
@@ -322,6 +322,15 @@ public:
     int check_precision() const { return precision_ == 0 ? 0 : (precision_ == 1 || precision_ == 3) ? 1 : 2; }
     // Tile skipping (see ccnn_set_skipnet): params NULL switches it off.
     int set_skipnet(const float* params, float threshold, int min_face, float scale_factor);
+    // Per-slot keep thresholds (CCNN_SKIP_SLOTS probabilities in (0, 1)). 0, or -1.
+    int set_skip_slot_thresholds(const float* thr)
+    {
+        for (int k = 0; k < CCNN_SKIP_SLOTS; ++k)
+            if (!(thr[k] > 0.f && thr[k] < 1.f)) return -1;
+        for (int k = 0; k < CCNN_SKIP_SLOTS; ++k) skip_threshold_[k] = thr[k];
+        batch_.key.clear();
+        return 0;
+    }
     // Keep window of the tile mask: scale slots within slot_radius of a level's own, cells
     // within cell_margin of its windows' centres (defaults 0 and 1). 0, or -1.
     int set_skip_window(int slot_radius, int cell_margin)
@@ -433,7 +442,7 @@ private:
     float* d_mw_[2] = {nullptr, nullptr};                     // MfmaW tables of stages 2, 3
     int precision_ = 0;
     float* d_skip_ = nullptr;      // skip-net weights (FP16-rounded), nullptr when off
-    float skip_threshold_ = 0.5f;
+    float skip_threshold_[CCNN_SKIP_SLOTS] = {};   // keep probability threshold per scale slot
     int skip_min_face_ = 0;
     float skip_sf_ = 0.f;
     int skip_srad_ = 0;

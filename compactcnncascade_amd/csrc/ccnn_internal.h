@@ -4,6 +4,8 @@
 #include <cstdint>
 #include <cstddef>
 
+#include "ccnn.h"
+
 namespace ccnn {
 
 // ---------------------------------------------------------------------------------
@@ -184,6 +186,12 @@ struct CheckParams {
     int adapt;            // adapt_min_neighbors (m-1 when scale > 0.7 and m > 1)
     int double_check;
     int equalize, reflection;
+};
+
+// Logit thresholds of the tile mask, one per scale slot (a tile of a level whose slot is s
+// runs when its largest logit reaches zt[s]).
+struct SkipThresholds {
+    float zt[CCNN_SKIP_SLOTS];
 };
 
 }  // namespace ccnn

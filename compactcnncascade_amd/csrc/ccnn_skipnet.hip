@@ -235,11 +235,12 @@ __global__ void __launch_bounds__(256) k_sn_adam(float* __restrict__ p, const fl
 // Keep flag of every response tile of a batch: the largest logit of the slots within
 // `srad` of L.scl over the frame cells that hold the centres of the tile's windows (level
 // position 4 * x + 11.5, 4 * y + 13.5, divided by the scale), widened by `margin` cells,
-// against the threshold's logit. Levels whose scale index has no slot are kept. One wave
+// against the logit threshold of slot L.scl. Levels whose scale index has no slot are kept. One wave
 // per tile: the lanes share the tile's (slot, cell) pairs and reduce their maxima.
 __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restrict__ levels, int nlevels, int ntiles,
                                                       int TX, int TY, const float* __restrict__ logits,
-                                                      const SnFrame* __restrict__ fr, float zt, int srad, int margin,
+                                                      const SnFrame* __restrict__ fr, SkipThresholds th, int srad,
+                                                      int margin,
                                                       unsigned char* __restrict__ keep)
 {
     const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(256) k_sn_tile_mask(const LevelDesc* __restric
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) mz = fmaxf(mz, __shfl_xor(mz, off, 64));
-            k = mz >= zt;
+            k = mz >= th.zt[L.scl];
         }
     }
     if (lane == 0) keep[t] = (unsigned char)k;
@@ -387,18 +388,19 @@ struct SnRun {
 
 hipError_t launch_skipnet(const float* gray, const FrameDesc* ft, const SnFrame* fr, int nf, int cells,
                           const float* par, float* x0, float* a1, float* a2, float* z, const LevelDesc* levels, int nl,
-                          int ntiles, int nl1tiles, int shape, float threshold, int srad, int margin,
+                          int ntiles, int nl1tiles, int shape, const float* threshold, int srad, int margin,
                           unsigned char* keep, unsigned char* l1_keep, hipStream_t st)
 {
     if (cells <= 0 || ntiles <= 0) return hipSuccess;
     const TileShape t = tile_shape(shape);
-    const float zt = std::log(threshold / (1.f - threshold));
+    SkipThresholds th;
+    for (int k = 0; k < CCNN_SKIP_SLOTS; ++k) th.zt[k] = std::log(threshold[k] / (1.f - threshold[k]));
     auto blocks = [](long long k) { return (unsigned)((k + 255) / 256); };
     k_sn_down8<<<blocks(cells), 256, 0, st>>>(gray, ft, fr, nf, cells, x0);
     k_sn_conv<1, 8><<<blocks(cells), 256, 0, st>>>(x0, par + oW1, par + oB1, a1, fr, nf, cells, 1);
     k_sn_conv<8, 8><<<blocks(cells), 256, 0, st>>>(a1, par + oW2, par + oB2, a2, fr, nf, cells, 1);
     k_sn_conv<8, S><<<blocks(cells), 256, 0, st>>>(a2, par + oW3, par + oB3, z, fr, nf, cells, 0);
-    k_sn_tile_mask<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, zt, srad, margin, keep);
+    k_sn_tile_mask<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(levels, nl, ntiles, t.tx, t.ty, z, fr, th, srad, margin, keep);
     k_sn_l1_mask<<<blocks(nl1tiles), 256, 0, st>>>(levels, nl, nl1tiles, t.ax, t.ay, t.tx, t.ty, keep, l1_keep);
     return hipGetLastError();
 }

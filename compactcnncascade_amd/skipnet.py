@@ -281,19 +281,96 @@ def choose_operating_points(sweep, target=0.995):
 
 
 def apply_operating_point(det, params, meta, which="recall"):
-    """Switches tile skipping on in a detector at one of the model's operating points."""
+    """Switches tile skipping on in a detector at one of the model's operating points (a
+    threshold for every slot, or one per slot)."""
     op = meta["operating_points"][which]
-    det.set_skipnet(params, op["threshold"], meta["min_face"], meta["scale_factor"])
+    det.set_skipnet(params, op.get("threshold", 0.5), meta["min_face"], meta["scale_factor"])
+    if "thresholds" in op:
+        det.set_skip_slot_thresholds(op["thresholds"])
     det.set_skip_window(op["slot_radius"], op["cell_margin"])
     return op
 
 
-def validate(api, params, meta, n_fhd=48, n_uhd=12):
+def choose_slot_thresholds(api, params, sets, window, start, target=0.995,
+                           grid=(0.9, 0.7, 0.5, 0.4, 0.3, 0.2, 0.15, 0.1, 0.07, 0.05, 0.03), slots=range(12),
+                           chunk=16, log=None):
+    """Per-slot keep thresholds, greedily: every slot starts at `start` (a threshold whose
+    recall meets the target everywhere); then slot by slot, in the order given (the
+    largest levels first), the largest threshold of `grid` that keeps the recall on every
+    validation set at `target` or above is taken. Returns (thresholds, recall per set,
+    tiles run per set)."""
+    from compactcnncascade_amd import recall
+    dets = {}
+    for name, frames in sets.items():
+        H, W = frames[0].shape[:2]
+        det = api.FaceDetector()
+        if det.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
+            raise RuntimeError("FaceDetector.Init failed")
+        base = []
+        for k in range(0, len(frames), chunk):
+            base += [f for _, f in det.DetectBatch(images=frames[k:k + chunk], capacity=4096)]
+        det.set_skipnet(params, 0.5, MIN_FACE, SCALE_FACTOR)
+        det.set_skip_window(*window)
+        dets[name] = (det, base)
+
+    def run(thr):
+        rc, run_ = {}, {}
+        for name, frames in sets.items():
+            det, base = dets[name]
+            det.set_skip_slot_thresholds(thr)
+            got, tiles, kept = [], 0, 0
+            for k in range(0, len(frames), chunk):
+                got += [f for _, f in det.DetectBatch(images=frames[k:k + chunk], capacity=4096)]
+                a, b = det.skip_stats()
+                tiles += a
+                kept += b
+            rc[name] = recall.recall(base, got)[0]
+            run_[name] = kept / max(1, tiles)
+        return rc, run_
+
+    thr = [float(start)] * SLOTS
+    rc, run_ = run(thr)
+    for s in slots:
+        for t in grid:
+            if t <= thr[s]:
+                break
+            trial = list(thr)
+            trial[s] = t
+            r2, u2 = run(trial)
+            if all(v >= target for v in r2.values()):
+                thr, rc, run_ = trial, r2, u2
+                break
+        if log:
+            log("slot %d: threshold %.2f, recall %s, tiles run %s" % (s, thr[s], rc, run_))
+    return thr, {k: round(v, 5) for k, v in rc.items()}, {k: round(v, 4) for k, v in run_.items()}
+
+
+def validate(api, params, meta, n_fhd=48, n_uhd=12, log=None):
     """Sweeps the operating points of a trained network on the validation frames and
-    records the sweep and the chosen points in its meta."""
+    records the sweep and the chosen points in its meta: a single threshold for all slots
+    over keep windows, then per-slot thresholds with the window of the best single point
+    ('recall', when it runs fewer tiles; the single-threshold one stays as
+    'recall_uniform')."""
     sets = validation_frames(n_fhd, n_uhd)
-    sweep = sweep_operating_points(api, to_fp16(params), sets)
-    meta["operating_points"] = choose_operating_points(sweep)
+    p16 = to_fp16(params)
+    sweep = sweep_operating_points(api, p16, sets)
+    ops = choose_operating_points(sweep)
+    if ops["recall"]:
+        uni = ops["recall"]
+        thr, rc, run_ = choose_slot_thresholds(api, p16, sets, (uni["slot_radius"], uni["cell_margin"]),
+                                               uni["threshold"], target=ops["target"], log=log)
+        row = next(r for r in sweep if r["threshold"] == uni["threshold"] and r["slot_radius"] == uni["slot_radius"]
+                   and r["cell_margin"] == uni["cell_margin"])
+        uni_run = sum(row[k]["run"] for k in sets) / len(sets)
+        slot_run = sum(run_.values()) / len(run_)
+        per_slot = {"thresholds": [round(t, 4) for t in thr], "slot_radius": uni["slot_radius"],
+                    "cell_margin": uni["cell_margin"], "validation_recall": rc, "validation_run": run_}
+        if slot_run < uni_run:
+            ops["recall_uniform"] = uni
+            ops["recall"] = per_slot
+        else:
+            ops["recall_per_slot"] = per_slot
+    meta["operating_points"] = ops
     meta["validation"] = {"frames": {k: len(v) for k, v in sets.items()}, "seeds": {"1920x1080": 800000,
                                                                                    "3840x2160": 810000},
                           "sweep": sweep}
@@ -381,7 +458,7 @@ def main():
         meta = {"min_face": MIN_FACE, "scale_factor": SCALE_FACTOR, "slots": SLOTS, "frames": args.frames,
                 "workload": TRAIN_WORKLOAD, "epochs": args.epochs, "pos_weight": args.pos_weight,
                 "loss": [float(v) for v in losses], "train_seconds": train_s}
-    meta = validate(api, p, meta, args.val_fhd, args.val_uhd)
+    meta = validate(api, p, meta, args.val_fhd, args.val_uhd, log=lambda m: print(m, file=sys.stderr, flush=True))
     save(args.out, p, meta)
     print(json.dumps(meta["operating_points"]), flush=True)
 

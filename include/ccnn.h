@@ -166,6 +166,10 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
  * cells within cell_margin of its windows' centres, reaches the threshold. Defaults 0
  * and 1; slot_radius < CCNN_SKIP_SLOTS, cell_margin <= 16. 0, or -1. */
 int ccnn_set_skip_window(ccnn_detector* d, int slot_radius, int cell_margin);
+/* Keep threshold per scale slot (no counterpart in the reference): CCNN_SKIP_SLOTS
+ * probabilities in (0, 1) replacing ccnn_set_skipnet's single threshold (which sets all
+ * slots). 0, or -1. */
+int ccnn_set_skip_slot_thresholds(ccnn_detector* d, const float* thresholds);
 /* Stage-1 response tiles in the last batch and how many of them ran (read from the keep
  * flags in device memory: a synchronous copy). 0, or -1. */
 int ccnn_skip_stats(ccnn_detector* d, long long* tiles, long long* kept);
