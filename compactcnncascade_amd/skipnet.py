@@ -347,10 +347,10 @@ def choose_slot_thresholds(api, params, sets, window, start, target=0.995,
 
 def validate(api, params, meta, n_fhd=48, n_uhd=12, log=None):
     """Sweeps the operating points of a trained network on the validation frames and
-    records the sweep and the chosen points in its meta: a single threshold for all slots
-    over keep windows, then per-slot thresholds with the window of the best single point
-    ('recall', when it runs fewer tiles; the single-threshold one stays as
-    'recall_uniform')."""
+    records the sweep and the chosen points in its meta: 'recall' and 'fast' with a single
+    threshold for all slots over keep windows, and 'recall_per_slot', per-slot thresholds
+    with the window of 'recall' (greedy on the same frames: it runs fewer tiles there but
+    fits those frames, so its held-out recall is lower)."""
     sets = validation_frames(n_fhd, n_uhd)
     p16 = to_fp16(params)
     sweep = sweep_operating_points(api, p16, sets)
@@ -362,14 +362,9 @@ def validate(api, params, meta, n_fhd=48, n_uhd=12, log=None):
         row = next(r for r in sweep if r["threshold"] == uni["threshold"] and r["slot_radius"] == uni["slot_radius"]
                    and r["cell_margin"] == uni["cell_margin"])
         uni_run = sum(row[k]["run"] for k in sets) / len(sets)
-        slot_run = sum(run_.values()) / len(run_)
-        per_slot = {"thresholds": [round(t, 4) for t in thr], "slot_radius": uni["slot_radius"],
-                    "cell_margin": uni["cell_margin"], "validation_recall": rc, "validation_run": run_}
-        if slot_run < uni_run:
-            ops["recall_uniform"] = uni
-            ops["recall"] = per_slot
-        else:
-            ops["recall_per_slot"] = per_slot
+        ops["recall_per_slot"] = {"thresholds": [round(t, 4) for t in thr], "slot_radius": uni["slot_radius"],
+                                  "cell_margin": uni["cell_margin"], "validation_recall": rc, "validation_run": run_,
+                                  "validation_run_single_threshold": round(uni_run, 4)}
     meta["operating_points"] = ops
     meta["validation"] = {"frames": {k: len(v) for k, v in sets.items()}, "seeds": {"1920x1080": 800000,
                                                                                    "3840x2160": 810000},
@@ -388,11 +383,12 @@ def evaluate(api, params, meta, n=256, chunk=32, sizes=((1920, 1080), (3840, 216
     for (W, H) in sizes:
         name = "%dx%d" % (W, H)
         dets = {}
-        for key in ("plain", "recall", "fast"):
+        points = [k for k in ("recall", "fast", "recall_per_slot") if meta["operating_points"].get(k)]
+        for key in ["plain"] + points:
             d = api.FaceDetector()
             if d.Init(api.Param(max_image_size=(W, H), min_face_height=MIN_FACE, scale_factor=SCALE_FACTOR)) != 0:
                 raise RuntimeError("FaceDetector.Init failed")
-            if key != "plain" and meta["operating_points"].get(key):
+            if key != "plain":
                 apply_operating_point(d, params, meta, key)
             dets[key] = d
         acc = {k: {"paired": 0, "ref": 0, "test": 0, "tiles": 0, "run": 0, "seconds": 0.0} for k in dets}
@@ -408,14 +404,14 @@ def evaluate(api, params, meta, n=256, chunk=32, sizes=((1920, 1080), (3840, 216
                     a, b = d.skip_stats()
                     acc[key]["tiles"] += a
                     acc[key]["run"] += b
-                for key in ("recall", "fast"):
+                for key in points:
                     _, paired, nref, ntest = recall.recall(res["plain"], res[key])
                     acc[key]["paired"] += paired
                     acc[key]["ref"] += nref
                     acc[key]["test"] += ntest
                 log("%s: %d/%d frames" % (name, c0 + len(frames), n))
         row = {"frames": n, "plain_fps": round(n / acc["plain"]["seconds"], 1)}
-        for key in ("recall", "fast"):
+        for key in points:
             a = acc[key]
             row[key] = {"recall": round(a["paired"] / max(1, a["ref"]), 5), "paired": [a["paired"], a["ref"], a["test"]],
                         "tiles_run_frac": round(a["run"] / max(1, a["tiles"]), 4),
