@@ -49,6 +49,10 @@ FP8_PEAK_TFLOPS = 5000.0
 VALU_EXACT_PEAK_TOPS = 78.6
 STAGE2_MAC = 278496
 STAGE3_MAC = 276672
+# matrix-core stages 2/3: per network evaluation L1 takes 33 row tiles x 2 column tiles and
+# L2 7 row tiles x 3 sum pairs of 16x16x32 products, each as three MFMAs (two-term
+# operands): 261 MFMAs of 16 x 16 x 32 x 2 FLOP
+MFMA_FLOP_PER_EVAL = 261 * 16 * 16 * 32 * 2
 
 
 def progress(msg):
@@ -126,6 +130,16 @@ def secondary_fps(api, torch, names, per_name, steps, facial=False, precisions=(
     out = {"frames": len(frames), "sizes": sorted({"%dx%d" % (f.shape[1], f.shape[0]) for f in frames}),
            "min_face": cfg["min_face"], "scale_factor": cfg["scale_factor"]}
     base = None
+    evals = None
+    if any(p in (3, 4) for p in precisions):
+        # the stage-2/3 evaluations the check kernel executes over these frames (the exact
+        # stage 1 of precisions 3/4 gives the same candidates), from per-frame traces
+        det = detector(api, W, H, cfg["min_face"], cfg["max_face"], cfg["scale_factor"], 0, mode, facial)
+        evals = 0
+        for (ptr, w, h, c, st) in df:
+            det.DetectDevice(ctypes.c_void_p(ptr), w, h, c, st)
+            r = det.last_trace()["res"]
+            evals += int(r[:, 6].sum() + r[:, 7].sum()) if len(r) else 0
     for p in precisions:
         det = detector(api, W, H, cfg["min_face"], cfg["max_face"], cfg["scale_factor"], p, mode, facial)
         dt, res = timed_batches(torch, det, df, steps)
@@ -133,6 +147,21 @@ def secondary_fps(api, torch, names, per_name, steps, facial=False, precisions=(
         leg = {"fps": round(len(frames) / dt, 2), "faces": int(sum(len(r) for r in res))}
         if mode == 1:
             leg["launches_per_batch"] = det.batch_stats()[0]
+        if mode == 1 and p in (3, 4) and evals:
+            # matrix-core roofline of the check kernel: one profiled batch (kernels without
+            # the graph, HIP events between them)
+            det.set_profiling(True)
+            det.DetectBatch(device_frames=df)
+            bt = det.last_batch_timing()
+            det.set_profiling(False)
+            flop = evals * MFMA_FLOP_PER_EVAL
+            tf = flop / (bt["check"] * 1e-3) / 1e12
+            peak = BF16_PEAK_TFLOPS if p == 3 else FP8_PEAK_TFLOPS
+            leg["mfma_roofline"] = {"kernel": "k_check", "bound": "mfma", "evaluations": evals,
+                                    "mfma_tflop": round(flop / 1e12, 4), "ms": round(bt["check"], 4),
+                                    "achieved": round(tf, 2), "peak": peak, "unit": "TFLOP/s",
+                                    "frac": round(tf / peak, 5),
+                                    "dtype": "bf16" if p == 3 else "fp8 e4m3"}
         if p == 0:
             base = res
         elif base is not None:

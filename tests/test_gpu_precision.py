@@ -101,7 +101,8 @@ def test_reduced_precision_recall_against_fp32(api, which):
         res[prec] = [f for _, f in d.DetectBatch(images=frames, capacity=20000)]
     # a reduced precision moves scores near the thresholds, and a window that crosses one
     # changes which boxes Merger averages: the floors sit below the measured recall
-    floors = {api.FP16_BF16: 0.9, api.FP16_FP8: 0.9, api.FP32_BF16: 0.9, api.FP32_FP8: 0.9}
+    # the exact stage 1 with BF16 stages 2/3 keeps every FP32 box here (measured 1.0)
+    floors = {api.FP16_BF16: 0.9, api.FP16_FP8: 0.9, api.FP32_BF16: 0.995, api.FP32_FP8: 0.9}
     for prec, floor in floors.items():
         rc, paired, nref, ntest = recall.recall(res[api.FP32], res[prec])
         print("%s precision %d: recall %.4f (%d of %d FP32 detections paired at IoU >= 0.99; %d detections)"
