@@ -230,7 +230,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
         s1_n += P.s1_total;
         tiles += P.ntiles;
         l1tiles += P.nl1tiles;
-        quads += (long long)((f.gw + 3) / 4) * f.gh;
+        quads += (long long)((f.gw + 15) / 16) * f.gh;
         snf[fi].h8 = (f.gh + 7) / 8;
         snf[fi].w8 = (f.gw + 7) / 8;
         snf[fi].off = snf[fi].pos_off = (int)cells;
@@ -333,11 +333,12 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
         }
         F.gray_off = (long long)goff[fi];
         F.gp = gpitch[fi];
-        F.qw = (F.w + 3) / 4;
+        F.qw = (F.w + 15) / 16;
         F.quads_off = (int)q0;
         q0 += (long long)F.qw * F.h;
         const uintptr_t al = F.ch == 4 ? 16 : 4;
-        F.aligned = ((uintptr_t)F.src % al) == 0 && (uintptr_t)F.step % al == 0;
+        F.aligned = (((uintptr_t)F.src % al) == 0 && (uintptr_t)F.step % al == 0 ? 1 : 0) |
+                    (((uintptr_t)F.src % 16) == 0 && (uintptr_t)F.step % 16 == 0 ? 2 : 0);
         B.h_ftab[fi] = F;
     }
     HIP_OK(hipMemcpyAsync(B.d_ftab, B.h_ftab, sizeof(FrameDesc) * nf, hipMemcpyHostToDevice, B.st));

@@ -115,9 +115,9 @@ struct FrameDesc {
     int ow, oh, ostep;
     long long gray_off;   // offset (floats) of its FP32 gray image in the batch's gray buffer
     int gp;               // gray pitch (floats)
-    int qw;               // quads (4 pixels) per row
-    int quads_off;        // first quad of the frame in the batch's quad numbering
-    int aligned;          // 4-byte (16-byte for BGRA) aligned rows: vector loads
+    int qw;               // groups of 16 pixels per row
+    int quads_off;        // first group of the frame in the batch's group numbering
+    int aligned;          // bit 0: 4-byte (16-byte for BGRA) aligned rows; bit 1: 16-byte aligned rows
 };
 
 // Stage 1 runs in two launches. The first computes the four L1-sum maps (conv, act,

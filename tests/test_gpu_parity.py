@@ -43,13 +43,14 @@ def test_gray_bitexact(api, oracle):
     # 4-pixel-vector path (4-byte aligned rows; 16-byte for BGRA) with full and partial
     # last quads, and the byte path for other row steps
     for shape in [(61, 77, 3), (33, 45, 4), (29, 31), (1080, 1920, 3), (37, 84, 3), (37, 83, 3), (21, 80, 4),
-                  (21, 81, 4), (19, 64, 1), (19, 66, 1), (5, 3, 3)]:
+                  (21, 81, 4), (21, 84, 4), (19, 64, 1), (19, 66, 1), (5, 3, 3), (7, 16, 3), (9, 12, 4)]:
         img = rng.integers(0, 256, size=shape, dtype=np.uint8)
         print("gray", shape, flush=True)
         assert np.array_equal(bits(api.stage_gray(img)), bits(oracle.gray(img)))
     # a padded row step (a view into a wider frame)
     wide = rng.integers(0, 256, size=(40, 100, 3), dtype=np.uint8)
-    for sub in [wide[:, :70], wide[3:, 5:90]]:
+    wide16 = rng.integers(0, 256, size=(40, 160, 3), dtype=np.uint8)    # 480-byte rows
+    for sub in [wide[:, :70], wide[3:, 5:90], wide16[:, :70], wide16[2:, 16:101], wide16[:, 32:157]]:
         assert np.array_equal(bits(api.stage_gray(sub)), bits(oracle.gray(np.ascontiguousarray(sub))))
 
 
@@ -406,6 +407,24 @@ def test_batch_matches_single_frame_results(api, golden):
     d3 = api.FaceDetector()
     assert d3.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
     singles = [d3.Detect(f) for f in frames]
+    for (cb, fb), (cs, fs) in zip(batch, singles):
+        assert cb == cs and np.array_equal(fb, fs)
+    # the batch gray kernel's 16-pixel groups: BGRA rows of 16-byte multiples whose width
+    # ends in a partial group, and a BGR view with a padded (16-byte multiple) row step,
+    # against one Detect at a time (whose gray kernel takes 4 pixels per thread)
+    rng = np.random.default_rng(7)
+    bgra = np.ascontiguousarray(np.concatenate([frames[0][:300, :292], rng.integers(0, 256, (300, 292, 1),
+                                                                                     dtype=np.uint8)], axis=2))
+    wide = np.zeros((360, 512, 3), np.uint8)
+    wide[:, :470] = frames[0][:360, :470]
+    view = wide[:, :470]
+    extra = [bgra, view, bgra]
+    d4 = api.FaceDetector()
+    assert d4.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+    batch = d4.DetectBatch(images=extra)
+    d5 = api.FaceDetector()
+    assert d5.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+    singles = [d5.Detect(f) for f in extra]
     for (cb, fb), (cs, fs) in zip(batch, singles):
         assert cb == cs and np.array_equal(fb, fs)
 
