@@ -597,12 +597,11 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     using G = s1::RespGeo<TX, TY>;
     constexpr int P1W = G::P1W, P1H = G::P1H, P2W = G::P2W, P1N = G::P1N, P2N = G::P2N;
     static_assert(TX * TY <= S1_THREADS, "one response value per thread");
-    __shared__ __align__(16) float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums
-    __shared__ __align__(16) float s_p2[N::M2 * P2N];     // L2 maps; later the tail weights
+    __shared__ __align__(16) float s_s1[N::M1 * P1N];     // L1 sums; later the L2 sums and tail weights
     // the L3, hidden- and output-layer weights, which the per-response tail reads from
     // LDS; fetched into registers now, stored over the consumed L2 maps later
     constexpr int WT = N::SIZE - N::oK3, WPER = (WT + S1_THREADS - 1) / S1_THREADS;
-    static_assert(WT <= N::M2 * P2N, "tail weights fit in the L2 map space");
+
 
     const int tid = threadIdx.x;
     float wt[WPER];
@@ -656,92 +655,103 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     }
     __syncthreads();
 
-    // L2: two conv_3x3 chains per sum + lrelu_bn_max
+    // L2, one pooled L2 position per thread: the eight maps (two conv_3x3 chains per L1
+    // sum + lrelu_bn_max), then their neighbour sums, held in registers until every thread
+    // has read the L1 sums and then written over them
     const float* k2 = W + N::oK2;
     const float* b2 = W + N::oB2;
-    for (int it = tid; it < N::M1 * nq; it += S1_THREADS) {
-        const int i = (int)(((float)it + 0.5f) * rnq), r = it - i * nq;
+    constexpr int RP = (P2W * G::P2H + S1_THREADS - 1) / S1_THREADS;   // positions per thread
+    float sv[RP][N::M2];
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const int r = tid + k * S1_THREADS;
+        if (r >= nq) break;
         const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
-        const int p = qy * P2W + qx;
-        const float* src = s_s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
-        float win[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
-        if constexpr (H) {
-            // pairs (c, c+1) of each window row, and (c3, 0) for the last tap of the
-            // right-hand quads; per map and row the tap pairs (k0, k1), (k2, 0)
-            h2_t hp[4][3], hz[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) hp[r][c] = pk16(win[r][c], win[r][c + 1]);
-                hz[r] = pk16(win[r][3], 0.f);
-            }
-            const uint32_t* kh = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL2;
-#pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int m = 2 * i + half;
-                float a[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int dx = q & 1, dy = q >> 1;
-                    float acc = 0.f;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        acc = dot2(hp[dy + r][dx], as_h2(kh[m * 6 + 2 * r]), acc);
-                        acc = dot2(dx ? hz[dy + r] : hp[dy + r][2], as_h2(kh[m * 6 + 2 * r + 1]), acc);
-                    }
-                    a[q] = lrelu_bn_fma(acc, b2, m, N::M2);
-                }
-                s_p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
-            }
-        } else {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            const int m = 2 * i + half;
-            const float* k = k2 + m * 9;
-            float a[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int dx = q & 1, dy = q >> 1;
-                float acc = win[dy][dx] * k[0];
-#pragma unroll
-                for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * k[j];
-                a[q] = lrelu_bn(acc, b2, m, N::M2);
-            }
-            s_p2[m * P2N + p] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
-        }
-        }
-    }
-    __syncthreads();
-    // L2 sums overwrite the (consumed) L1 sums, with rows P2S floats apart: an odd multiple
-    // of 8, so that the rows of an 8 x 8 block of responses (below) start in distinct
-    // groups of 8 LDS banks and the L3 window reads of a wave are free of bank conflicts
-    constexpr int P2S = (P2W + 7) / 16 * 16 + 8, P2NS = P2S * G::P2H;
-    static_assert(P2S >= P2W && N::M2 * P2NS <= N::M1 * P1N, "L2 sums fit in the L1-sum space");
-    float* s_s2 = s_s1;
-    for (int r = tid; r < nq; r += S1_THREADS) {
-        const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
-        const int p = qy * P2W + qx, ps = qy * P2S + qx;
         float v[N::M2];
 #pragma unroll
-        for (int m = 0; m < N::M2; ++m) v[m] = s_p2[m * P2N + p];
-        s_s2[ps] = v[0] + v[1];
+        for (int i = 0; i < N::M1; ++i) {
+            const float* src = s_s1 + i * P1N + (2 * qy) * P1W + 2 * qx;
+            float win[4][4];
 #pragma unroll
-        for (int i = 1; i < N::M2 - 1; ++i) s_s2[i * P2NS + ps] = v[i - 1] + v[i] + v[i + 1];
-        s_s2[(N::M2 - 1) * P2NS + ps] = v[N::M2 - 2] + v[N::M2 - 1];
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) win[r][c] = src[r * P1W + c];
+            if constexpr (H) {
+                // pairs (c, c+1) of each window row, and (c3, 0) for the last tap of the
+                // right-hand quads; per map and row the tap pairs (k0, k1), (k2, 0)
+                h2_t hp[4][3], hz[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) hp[r][c] = pk16(win[r][c], win[r][c + 1]);
+                    hz[r] = pk16(win[r][3], 0.f);
+                }
+                const uint32_t* kh = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL2;
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    const int m = 2 * i + half;
+                    float a[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int dx = q & 1, dy = q >> 1;
+                        float acc = 0.f;
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            acc = dot2(hp[dy + r][dx], as_h2(kh[m * 6 + 2 * r]), acc);
+                            acc = dot2(dx ? hz[dy + r] : hp[dy + r][2], as_h2(kh[m * 6 + 2 * r + 1]), acc);
+                        }
+                        a[q] = lrelu_bn_fma(acc, b2, m, N::M2);
+                    }
+                    v[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+                }
+            } else {
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    const int m = 2 * i + half;
+                    const float* kk = k2 + m * 9;
+                    float a[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int dx = q & 1, dy = q >> 1;
+                        float acc = win[dy][dx] * kk[0];
+#pragma unroll
+                        for (int j = 1; j < 9; ++j) acc = acc + win[dy + j / 3][dx + j % 3] * kk[j];
+                        a[q] = lrelu_bn(acc, b2, m, N::M2);
+                    }
+                    v[m] = fmaxf(fmaxf(a[0], a[1]), fmaxf(a[2], a[3]));
+                }
+            }
+        }
+        sv[k][0] = v[0] + v[1];
+#pragma unroll
+        for (int i = 1; i < N::M2 - 1; ++i) sv[k][i] = v[i - 1] + v[i] + v[i + 1];
+        sv[k][N::M2 - 1] = v[N::M2 - 2] + v[N::M2 - 1];
     }
     __syncthreads();
-    float* s_wt = s_p2;
+    // the L2 sums over the (consumed) L1 sums, with rows P2S floats apart: an odd multiple
+    // of 8, so that the rows of an 8 x 8 block of responses (below) start in distinct
+    // groups of 8 LDS banks and the L3 window reads of a wave are free of bank conflicts;
+    // the tail weights after them
+    constexpr int P2S = (P2W + 7) / 16 * 16 + 8, P2NS = P2S * G::P2H;
+    float* s_s2 = s_s1;
+#pragma unroll
+    for (int k = 0; k < RP; ++k) {
+        const int r = tid + k * S1_THREADS;
+        if (r >= nq) break;
+        const int qy = (int)(((float)r + 0.5f) * rqw), qx = r - qy * qw;
+        const int ps = qy * P2S + qx;
+#pragma unroll
+        for (int m = 0; m < N::M2; ++m) s_s2[m * P2NS + ps] = sv[k][m];
+    }
+    float* s_wt = s_s1 + N::M2 * P2NS;
+    static_assert(P2S >= P2W && N::M2 * P2NS + WT + Stage1H::SIZE <= N::M1 * P1N,
+                  "L2 sums and tail weights fit in the L1-sum space");
 #pragma unroll
     for (int k = 0; k < WPER; ++k)
         if (tid + k * S1_THREADS < WT) s_wt[tid + k * S1_THREADS] = wt[k];
     // FP16 path: the L3 and hidden-layer tap pairs follow the FP32 tail weights in LDS
     uint32_t* s_wh = reinterpret_cast<uint32_t*>(s_wt + WT);
     if constexpr (H) {
-        static_assert(WT + Stage1H::SIZE - Stage1H::oL3 <= N::M2 * P2N, "FP16 tail weights fit");
         const uint32_t* src = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL3;
         for (int k = tid; k < Stage1H::SIZE - Stage1H::oL3; k += S1_THREADS) s_wh[k] = src[k];
     }
