@@ -78,6 +78,11 @@ void append_stage1_half(std::vector<float>& w)
             h[Stage1H::oL1 + m * 8 + 2 * r] = pack_half2(k[0], k[1]);
             h[Stage1H::oL1 + m * 8 + 2 * r + 1] = pack_half2(k[2], k[3]);
         }
+    for (int m = 0; m < N::M1; ++m) {
+        float sum = 0.f;
+        for (int t = 0; t < 16; ++t) sum += w[N::oK1 + m * 16 + t];
+        std::memcpy(&h[Stage1H::oS1 + m], &sum, 4);
+    }
     for (int m = 0; m < N::M2; ++m)
         for (int r = 0; r < 3; ++r) {
             const float* k = &w[N::oK2 + m * 9 + r * 3];

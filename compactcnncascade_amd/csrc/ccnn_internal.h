@@ -51,13 +51,16 @@ using Landmarks = NetShape<24, 4, 48, 96, 10, 11, 1, 384, 96, 13>;
 // FP16 tap pairs of the stage-1 network for the reduced-precision path, as 32-bit words
 // (two FP16 values each) after the FP32 weights: per L1 map and kernel row (k0, k1) and
 // (k2, k3); per L2 map and row (k0, k1) and (k2, 0); per L3 map and row (k0, k1) and
-// (k2, k3); per hidden unit (w0, w1) and (w2, w3).
+// (k2, k3); per hidden unit (w0, w1) and (w2, w3); then, as FP32 bits, the sum of each
+// L1 map's 16 FP32 taps (the FP16 L1 convolution runs on the window less its centre
+// value and adds centre x sum back in FP32).
 struct Stage1H {
     static constexpr int oL1 = 0;
     static constexpr int oL2 = oL1 + Stage1::M1 * Stage1::K1 * 2;
     static constexpr int oL3 = oL2 + Stage1::M2 * 3 * 2;
     static constexpr int oHL = oL3 + Stage1::M3 * Stage1::K3H * 2;
-    static constexpr int SIZE = oHL + Stage1::HL * 2;
+    static constexpr int oS1 = oHL + Stage1::HL * 2;
+    static constexpr int SIZE = oS1 + Stage1::M1;
 };
 
 // Weights of the stage-2/3 networks for the matrix-core path, expanded per lane of the

@@ -537,21 +537,27 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
             for (int c = 0; c < 5; ++c) win[r][c] = s_in[(2 * py + r) * AW + 2 * px + c];
         float pm[N::M1];
         if constexpr (H) {
-            // the window's adjacent pairs (c, c+1) of each row in FP16; tap pairs
-            // (k[r][0], k[r][1]) and (k[r][2], k[r][3]) of each map in Stage1H
+            // the window's adjacent pairs (c, c+1) of each row in FP16, less the centre
+            // value mu (the differences are small in smooth regions, so they round far
+            // closer than the raw values, and the taps' rounding error scales with them
+            // too); tap pairs (k[r][0], k[r][1]) and (k[r][2], k[r][3]) of each map in
+            // Stage1H; each map's sum starts from mu x its FP32 tap sum
+            const float mu = win[2][2];
             h2_t hp[5][4];
 #pragma unroll
             for (int r = 0; r < 5; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) hp[r][c] = pk16(win[r][c], win[r][c + 1]);
+                for (int c = 0; c < 4; ++c) hp[r][c] = pk16(win[r][c] - mu, win[r][c + 1] - mu);
             const uint32_t* kh = reinterpret_cast<const uint32_t*>(W + N::SIZE) + Stage1H::oL1;
+            const float* ks = W + N::SIZE + Stage1H::oS1;
 #pragma unroll
             for (int m = 0; m < N::M1; ++m) {
                 float a[4];
+                const float acc0 = mu * ks[m];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int dx = q & 1, dy = q >> 1;
-                    float acc = 0.f;
+                    float acc = acc0;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         acc = dot2(hp[dy + r][dx], as_h2(kh[m * 8 + 2 * r]), acc);

@@ -49,9 +49,10 @@ def test_stage1_fp16_error_vs_fp64(api, oracle):
         e32 = max_abs_error(api.stage1(lvl, precision=api.FP32), e64)
         e16 = max_abs_error(api.stage1(lvl, precision=api.FP16_BF16), e64)
         print("stage 1 level %s: max |error| vs FP64: FP32 %.3g, FP16 %.3g" % (lvl.shape, e32, e16), flush=True)
-        # responses lie in [-1.7159, 1.7159]
+        # responses lie in [-1.7159, 1.7159]; FP16 L1 on centred windows measures 0.013
+        # at most here (0.023 with the raw 8-bit-scale samples in FP16)
         assert e32 < 2e-5
-        assert e16 < 0.05
+        assert e16 < 0.02
 
 
 def test_stage23_bf16_fp8_error_vs_fp64(api, oracle):
@@ -101,8 +102,8 @@ def test_reduced_precision_recall_against_fp32(api, which):
         res[prec] = [f for _, f in d.DetectBatch(images=frames, capacity=20000)]
     # a reduced precision moves scores near the thresholds, and a window that crosses one
     # changes which boxes Merger averages: the floors sit below the measured recall
-    # the exact stage 1 with BF16 stages 2/3 keeps every FP32 box here (measured 1.0)
-    floors = {api.FP16_BF16: 0.9, api.FP16_FP8: 0.9, api.FP32_BF16: 0.995, api.FP32_FP8: 0.9}
+    # the BF16 stages 2/3 keep every FP32 box here (measured 1.0 with either stage 1)
+    floors = {api.FP16_BF16: 0.995, api.FP16_FP8: 0.9, api.FP32_BF16: 0.995, api.FP32_FP8: 0.9}
     for prec, floor in floors.items():
         rc, paired, nref, ntest = recall.recall(res[api.FP32], res[prec])
         print("%s precision %d: recall %.4f (%d of %d FP32 detections paired at IoU >= 0.99; %d detections)"
