@@ -108,9 +108,10 @@ int ccnn_set_batch_mode(ccnn_detector* d, int mode);
  *   CCNN_PRECISION_FP32       the exact FP32 path (default): results bit-identical to the
  *                             reference's CPU path;
  *   CCNN_PRECISION_FP16_BF16  stage 1 with FP16 convolution inputs and weights (products
- *                             summed in FP32, v_dot2_f32_f16); the stage-2/3 L1 and L2
- *                             layers as BF16 GEMMs on the matrix cores with FP32
- *                             accumulation;
+ *                             summed in FP32, v_dot2_f32_f16; L1 windows rounded less
+ *                             their centre sample, which is added back in FP32); the
+ *                             stage-2/3 L1 and L2 layers as BF16 GEMMs on the matrix
+ *                             cores with FP32 accumulation;
  *   CCNN_PRECISION_FP16_FP8   the same with FP8 (e4m3) operands for stages 2/3, scaled
  *                             per layer by powers of two;
  *   CCNN_PRECISION_FP32_BF16  the exact FP32 stage 1 with the BF16 stages 2/3;
