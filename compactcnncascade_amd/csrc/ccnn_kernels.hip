@@ -1047,6 +1047,7 @@ struct ChkSmem {
     alignas(16) float in[chk::IH * chk::IW];
     alignas(16) float s1[6 * chk::P1N];
     float w[chk::WL];
+    float w3[Stage3::SIZE];  // k_check: the stage-3 weights beside the stage-2 ones in w
     __device__ float* p2() { return in; }
     __device__ float* a3() { return in; }
     __device__ float* out() { return in + 24 * chk::RN; }
@@ -1174,12 +1175,12 @@ __device__ __forceinline__ f32x4_t mma_act(const float (&a)[8], const WOp<RP>& b
 __device__ __forceinline__ float pow2f(int e) { return __int_as_float((e + 127) << 23); }
 
 template <class N, int RP>
-__device__ __forceinline__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s, const int tid,
-                                const float* __restrict__ mw, int* s_red)
+__device__ __forceinline__ void check_l1l2_mfma(const float* __restrict__ in, ChkSmem& s,
+                                                const float* Wl, const int tid,
+                                                const float* __restrict__ mw, int* s_red)
 {
     using namespace chk;
     using Op = MOp<RP>;
-    const float* Wl = s.w;
     const int lane = tid & 63, wave = tid >> 6;
     const int row = lane & 15, kb = lane >> 4;
     // this lane's weight operands: L1 (two column tiles), L2 (three sum pairs)
@@ -1289,16 +1290,16 @@ __device__ __forceinline__ void check_l1l2_mfma(const float* __restrict__ in, Ch
 }
 
 template <class N, int RP = RP_EXACT>
-__device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkSmem& s, const int tid,
-                              const float* __restrict__ mw = nullptr, int* s_red = nullptr)
+__device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkSmem& s,
+                                              const float* Wl, const int tid,
+                                              const float* __restrict__ mw = nullptr, int* s_red = nullptr)
 {
     using namespace chk;
-    // the network's weights, staged in LDS (s.w); read the same for every thread in L1,
+    // Wl: the network's weights, staged in LDS; read the same for every thread in L1,
     // per thread from L2 on
-    const float* Wl = s.w;
     const unsigned ut = (unsigned)tid;
     if constexpr (RP != RP_EXACT) {
-        check_l1l2_mfma<N, RP>(in, s, tid, mw, s_red);
+        check_l1l2_mfma<N, RP>(in, s, Wl, tid, mw, s_red);
     } else {
     // L1 + sums: the first FULL positions one per lane (whole waves), the REM others by
     // one wave with a lane per 2x2-pool quad. Index arithmetic is unsigned throughout.
@@ -1416,7 +1417,11 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         }
     }
     __syncthreads();
-    // hidden layer
+    // hidden layer, each unit's value already multiplied by its output weight: unit
+    // u = HLS * gi + tt of response p goes to s.h()[p * HLP + u], so that the output
+    // layer's serial sum over u reads its terms with 16-byte loads issued together
+    constexpr int HLP = (N::HL + 3) / 4 * 4;
+    static_assert(HLP * RN <= 6 * P1N, "the output-layer terms fit the s1 region");
     for (unsigned it = ut; it < N::HL * RN; it += CHECK_THREADS) {
         const unsigned u = it / RN, p = it - u * RN;
         const unsigned gi = u / N::HLS, tt = u - gi * N::HLS;
@@ -1426,18 +1431,23 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         c = c * Wl[N::oHT + gi];
         c = rtanh(c);
         c = (c + 1.f) * 0.5f;
-        s.h()[u * RN + p] = Wl[N::oHBW + tt] * c + Wl[N::oHBB + tt];
+        const float hv = Wl[N::oHBW + tt] * c + Wl[N::oHBB + tt];
+        s.h()[p * HLP + u] = hv * Wl[N::oOW + tt * N::HLG + gi];
     }
     __syncthreads();
-    // output layer
+    // output layer: the products in unit order, summed serially (the first term alone,
+    // then term + sum: IEEE addition commutes, so the order of the sum is what counts)
     if (tid < RN) {
-        float o = 0.f;
-        for (int gi = 0; gi < N::HLG; ++gi)
-            for (int tt = 0; tt < N::HLS; ++tt) {
-                const float wv = Wl[N::oOW + tt * N::HLG + gi];
-                if (gi == 0 && tt == 0) o = s.h()[tid] * wv;
-                else o = s.h()[(N::HLS * gi + tt) * RN + tid] * wv + o;
-            }
+        float t[HLP];
+        const float4* src = reinterpret_cast<const float4*>(s.h() + tid * HLP);
+#pragma unroll
+        for (int q = 0; q < HLP / 4; ++q) {
+            const float4 v = src[q];
+            t[4 * q] = v.x; t[4 * q + 1] = v.y; t[4 * q + 2] = v.z; t[4 * q + 3] = v.w;
+        }
+        float o = t[0];
+#pragma unroll
+        for (int u = 1; u < N::HL; ++u) o = t[u] + o;
         float c = o + Wl[N::oOB];
         c = c * Wl[N::oOB + 1];
         c = rtanh(c);
@@ -1447,17 +1457,35 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
     __syncthreads();
 }
 
-// Copies n floats of weights into LDS with all of a thread's loads in flight together.
+// Copies n floats of weights into LDS with all of a thread's loads in flight together
+// (load_weights issues them, store_weights writes them once they have arrived).
+template <int n>
+struct WeightRegs {
+    static constexpr int PER = (n + CHECK_THREADS - 1) / CHECK_THREADS;
+    float v[PER];
+};
+
+template <int n>
+__device__ __forceinline__ void load_weights(WeightRegs<n>& r, const float* __restrict__ src, const int tid)
+{
+#pragma unroll
+    for (int k = 0; k < WeightRegs<n>::PER; ++k) r.v[k] = src[min(tid + k * CHECK_THREADS, n - 1)];
+}
+
+template <int n>
+__device__ __forceinline__ void store_weights(float* __restrict__ dst, const WeightRegs<n>& r, const int tid)
+{
+#pragma unroll
+    for (int k = 0; k < WeightRegs<n>::PER; ++k)
+        if (tid + k * CHECK_THREADS < n) dst[tid + k * CHECK_THREADS] = r.v[k];
+}
+
 template <int n>
 __device__ __forceinline__ void stage_weights(float* __restrict__ dst, const float* __restrict__ src, const int tid)
 {
-    constexpr int PER = (n + CHECK_THREADS - 1) / CHECK_THREADS;
-    float v[PER];
-#pragma unroll
-    for (int k = 0; k < PER; ++k) v[k] = src[min(tid + k * CHECK_THREADS, n - 1)];
-#pragma unroll
-    for (int k = 0; k < PER; ++k)
-        if (tid + k * CHECK_THREADS < n) dst[tid + k * CHECK_THREADS] = v[k];
+    WeightRegs<n> r;
+    load_weights<n>(r, src, tid);
+    store_weights<n>(dst, r, tid);
 }
 
 // Crop geometry of one candidate (cnn_detector_v3.cpp:1491-1556): the window around the
@@ -1491,24 +1519,34 @@ __device__ __forceinline__ Crop crop_geometry(int px, int py, float scale, int m
 // 357-368), then equalizeImage (image_proc.cpp:720-800: cumulative histogram and LUT
 // int((cf - cf[0]) * (255 / (cf[255] - cf[0])) + 0.5), skipped if the histogram is
 // flat). Each thread keeps its pixels (i = tid + k * CHECK_THREADS) in u[k]; when
-// remap is set, s.lut maps them to the equalized values.
+// remap is set, s.lut maps them to the equalized values. With stage set (a workgroup's
+// first item), the stage-2 and stage-3 weights W2 / W3 are loaded alongside the crop's
+// gathers and stored to s.w / s.w3 before the first barrier, so that both latencies
+// overlap.
 namespace chk {
 constexpr int PER = (CW * CH + CHECK_THREADS - 1) / CHECK_THREADS;
 }
 
-__device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr, bool equalize, ChkSmem& s,
-                             int* s_w, uint32_t (&u)[chk::PER], bool& remap, const int tid)
+__device__ __forceinline__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr, bool equalize,
+                                             ChkSmem& s, int* s_w, uint32_t (&u)[chk::PER], bool& remap,
+                                             const int tid, const float* __restrict__ W2,
+                                             const float* __restrict__ W3, bool stage)
 {
     using namespace chk;
     const bool copy = (cr.rcols == CW && cr.rrows == CH);
     const uint32_t sclx = ((uint32_t)cr.rcols << 12) / CW + 1;
     const uint32_t scly = ((uint32_t)cr.rrows << 12) / CH + 1;
     static_assert(CHECK_THREADS >= 256 && CHECK_THREADS % 64 == 0, "one histogram bin per thread");
-    if (tid < 256) s.hist()[tid] = 0;
-    __syncthreads();
+    WeightRegs<Stage2::SIZE> r2;
+    WeightRegs<Stage3::SIZE> r3;
+    if (stage) {
+        load_weights(r2, W2, tid);
+        load_weights(r3, W3, tid);
+    }
     // all of a thread's gathers are issued before any is used, so that their latencies
     // overlap; 32-bit offsets from the crop's corner keep one register per address
     const float* __restrict__ base = g + (size_t)cr.ry * gp + cr.rx;
+    float v[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int i = min(tid + k * CHECK_THREADS, CW * CH - 1);
@@ -1518,10 +1556,19 @@ __device__ void prepare_crop(const float* __restrict__ g, int gp, const Crop& cr
             px = min(((uint32_t)ix * sclx) >> 12, (uint32_t)(cr.rcols - 2));
             py = min(((uint32_t)iy * scly) >> 12, (uint32_t)(cr.rrows - 1));
         }
-        const float v = base[py * (uint32_t)gp + px];
-        // the pixel value in bits 0-7, its crop column and row above, for write_input
-        u[k] = ((uint32_t)(v + 0.5f) & 255u) | ((uint32_t)ix << 8) | ((uint32_t)iy << 16);
+        v[k] = base[py * (uint32_t)gp + px];
+        // its crop column and row in bits 8-15 and 16-31, for write_input
+        u[k] = ((uint32_t)ix << 8) | ((uint32_t)iy << 16);
     }
+    if (tid < 256) s.hist()[tid] = 0;
+    if (stage) {
+        store_weights(s.w, r2, tid);
+        store_weights(s.w3, r3, tid);
+    }
+    __syncthreads();
+    // the pixel value in bits 0-7
+#pragma unroll
+    for (int k = 0; k < PER; ++k) u[k] |= (uint32_t)(v[k] + 0.5f) & 255u;
     remap = false;
     if (equalize) {
 #pragma unroll
@@ -1625,6 +1672,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         __syncthreads();
         item = s_item;
     }
+    bool staged = false;
     while (item < items) {
         const int ci = nv == 2 ? item >> 1 : item;
         const int mod = nv == 2 ? item & 1 : 0;
@@ -1641,13 +1689,14 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         const float* __restrict__ g = gr.g;
         const int gp = gr.gp;
         const Crop cr = crop_geometry(cd.x * 4, cd.y * 4, scale, mod, gr.gw, gr.gh);
-        // stage-2 weights into LDS; the first barrier of prepare_crop orders them
-        stage_weights<Stage2::SIZE>(s.w, W2, tid);
+        // the stage-2 and stage-3 weights go to LDS with the workgroup's first item and
+        // stay there for its later items
         uint32_t u[PER];
         bool remap;
-        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w, u, remap, tid);
+        prepare_crop(g, gp, cr, P.equalize != 0, s, s_w, u, remap, tid, W2, W3, !staged);
+        staged = true;
         write_input(s, u, remap, false, P.reflection != 0, tid);
-        check_forward<Stage2, RP>(s.in, s, tid, MW2, s_w);
+        check_forward<Stage2, RP>(s.in, s, s.w, tid, MW2, s_w);
         if (tid == 0) {
             float d[RN];
 #pragma unroll
@@ -1677,10 +1726,8 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
                 if (hs3 && ci < hcap) hs3[slot] = make_int2(-1, __float_as_int(-10.f));
             }
         } else {
-        // stage-3 weights; the barrier closing write_input orders them
-        stage_weights<Stage3::SIZE>(s.w, W3, tid);
         write_input(s, u, false, true, P.reflection != 0, tid);
-        check_forward<Stage3, RP>(s.in, s, tid, MW3, s_w);
+        check_forward<Stage3, RP>(s.in, s, s.w3, tid, MW3, s_w);
         if (tid == 0) {
             float d[RN];
 #pragma unroll
@@ -2123,7 +2170,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_net23(const float* __restr
     stage_weights<N::SIZE>(s.w, W, tid);
     for (int i = tid; i < IW * IH; i += CHECK_THREADS) s.in[i] = src[i];
     __syncthreads();
-    check_forward<N, RP>(s.in, s, tid, MW, s_red);
+    check_forward<N, RP>(s.in, s, s.w, tid, MW, s_red);
     if (tid < RN) out[(size_t)blockIdx.x * RN + tid] = s.out()[tid];
 }
 
