@@ -236,3 +236,38 @@ def test_autotune_picks_a_shape_and_keeps_results(api, seqgold, workloads):
     for i, (n, faces) in enumerate(d2.DetectBatch(images=full)):
         assert n == len(g["faces"][i]), i
         check_frame(faces, g, i, "tuned")
+
+
+def test_check_grid_gives_the_same_detections(api, seqgold, workloads):
+    # the check kernel's grid changes only which workgroup evaluates which (candidate, crop
+    # variant): one workgroup per item (per-frame default), persistent workgroups taking
+    # several items each from a queue (per-frame 1 or 3 per CU; batch 1 or 8 per CU, 8 being
+    # more than fit on a CU at once); the 32 mixed-size frames (640x480 .. 3840x2160)
+    # through the graph batch and the per-frame pipelined batch, in the exact precision
+    # (against the reference's detections) and a matrix-core one (against itself)
+    name = "mixed_batch_x32"
+    spec, over, mx = WORKLOAD_SEQS[name]
+    frames = workloads[name]
+    g = seqgold[name]
+    p = api_param_of(seq_params(over, mx))
+    ref16 = {}
+    for frame_per_cu, batch_per_cu in ((0, 4), (1, 1), (3, 8)):
+        for prec in (api.FP32, api.FP16_BF16):
+            for mode in (1, 0):
+                d = api.FaceDetector()
+                assert d.Init(p) == 0
+                d.set_compute_precision(prec)
+                d.set_check_grid(frame_per_cu, batch_per_cu)
+                d.set_batch_mode(mode)
+                tag = "grid %d/%d precision %d mode %d" % (frame_per_cu, batch_per_cu, prec, mode)
+                print(tag, flush=True)
+                out = d.DetectBatch(images=frames)
+                if prec == api.FP32:
+                    for i, (n, faces) in enumerate(out):
+                        assert n == len(g["faces"][i]), (tag, i)
+                        check_frame(faces, g, i, tag)
+                else:
+                    got = [faces for _, faces in out]
+                    if mode not in ref16:
+                        ref16[mode] = got
+                    assert all(np.array_equal(a, b) for a, b in zip(got, ref16[mode])), tag
