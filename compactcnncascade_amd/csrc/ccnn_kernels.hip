@@ -1665,8 +1665,17 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
     const int items = min(meta[0], cap) * nv;
     if (hmeta && blockIdx.x == 0 && threadIdx.x == 0) hmeta[0] = meta[0];
     // queue (optional): a counter, zero at launch, from which the workgroups take items
-    // in turn (a persistent grid that balances the stage-3 tail); otherwise grid-stride
-    int item = blockIdx.x;
+    // in turn (a persistent grid that balances the stage-3 tail); otherwise grid-stride:
+    // round r covers items [r * G, r * G + n_r) with n_r = min(items - r * G, G), and in
+    // each round the workgroups are renumbered (xcd_tile over n_r) so that every XCD takes
+    // one contiguous run of items, whose candidates are neighbours with overlapping crops
+    // in that XCD's L2. The workgroups past n_r have no item in this round or any later.
+    int base = 0;
+    auto stride_item = [&]() {
+        const int nr = min(items - base, (int)gridDim.x);
+        return (int)blockIdx.x < nr ? base + xcd_tile(blockIdx.x, nr) : items;
+    };
+    int item = stride_item();
     if (queue) {
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
         __syncthreads();
@@ -1755,7 +1764,8 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
             item = s_item;
         } else {
             __syncthreads();
-            item += gridDim.x;
+            base += gridDim.x;
+            item = base < items ? stride_item() : items;
         }
     }
 }
