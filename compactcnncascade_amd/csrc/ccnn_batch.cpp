@@ -178,7 +178,8 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     // layout of the batch: each frame's gray image, level table, L1-sum planes, response
     // maps and stage-1 tiles follow the previous frame's
     std::vector<LevelDesc> lv;
-    std::vector<int2> order;
+    // tile tables: every frame's L1 tiles, then every frame's stage-1 tiles (s1order)
+    std::vector<int2> order, s1order;
     std::vector<int> lv_begin(nf + 1, 0), gpitch(nf);
     std::vector<size_t> goff(nf), foff(nf), poff(nf), roff(nf);
     std::vector<long long> key;
@@ -224,7 +225,10 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             L.s1_off += (int)s1_n;
             lv.push_back(L);
         }
-        for (const int2& o : P.l1_order) order.push_back(make_int2(o.x + lv_begin[fi], o.y));
+        for (int k = 0; k < P.nl1tiles; ++k)
+            order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
+        for (size_t k = P.nl1tiles; k < P.l1_order.size(); ++k)
+            s1order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
         roff[fi] = resp_n;
         resp_n += P.resp_total;
         s1_n += P.s1_total;
@@ -247,6 +251,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             return 1;
     }
     lv_begin[nf] = (int)lv.size();
+    order.insert(order.end(), s1order.begin(), s1order.end());
     const int nl = (int)lv.size();
     if (nl == 0) {
         // no level to run in any frame: no candidates

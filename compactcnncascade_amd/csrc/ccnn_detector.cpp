@@ -674,6 +674,10 @@ const Plan* Detector::plan_for(int gw, int gh, int ncap, int shape)
                 for (int tx = 0; tx < L.l1_tiles_x; ++tx) P->l1_order.push_back(make_int2(l, ty * L.l1_tiles_x + tx));
             }
         }
+    // then the stage-1 tiles in launch order, {level, tile within the level}
+    P->l1_order.reserve(l1tiles + tiles);
+    for (int l = 0; l < (int)P->levels.size(); ++l)
+        for (int t = 0; t < P->levels[l].tiles_x * P->levels[l].tiles_y; ++t) P->l1_order.push_back(make_int2(l, t));
     if (plans_.size() >= 8) {          // a handful of frame sizes stay cached
         for (size_t k = 0; k < plans_.size(); ++k) {
             bool used = plans_[k]->pins > 0;

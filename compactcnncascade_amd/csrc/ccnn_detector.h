@@ -173,7 +173,9 @@ struct Plan {
     int ntiles = 0, nchunks = 0, nl1tiles = 0;
     size_t resp_total = 0;
     size_t s1_total = 0;         // floats of the L1-sum planes of all levels
-    std::vector<int2> l1_order;  // L1 tiles {level, tile} in XCD-band order
+    // {level, tile within the level}: the nl1tiles L1 tiles in XCD-band order, then the
+    // ntiles stage-1 tiles in launch order
+    std::vector<int2> l1_order;
 };
 
 // Device and pinned-host state of one frame in flight. Three slots on three streams let the

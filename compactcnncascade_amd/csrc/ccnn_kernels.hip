@@ -597,6 +597,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
                                                        const float* __restrict__ s1buf, float* __restrict__ resp,
                                                        float t1, int* __restrict__ meta, Candidate* __restrict__ cands,
                                                        Candidate* __restrict__ hcands, int hcap,
+                                                       const int2* __restrict__ order,
                                                        const unsigned char* __restrict__ keep)
 {
     using N = Stage1;
@@ -614,9 +615,18 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 #pragma unroll
     for (int k = 0; k < WPER; ++k) wt[k] = W[N::oK3 + min(tid + k * S1_THREADS, WT - 1)];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int li = find_level<BY_TILE>(levels, nlevels, tile);
+    // order (optional): each tile's {level, tile within the level}, one load instead of
+    // the binary search over the level table (11 dependent loads for a 64-frame batch)
+    int li, t;
+    if (order) {
+        const int2 o = order[tile];
+        li = o.x;
+        t = o.y;
+    } else {
+        li = find_level<BY_TILE>(levels, nlevels, tile);
+        t = tile - levels[li].tile_off;
+    }
     const LevelDesc L = levels[li];
-    const int t = tile - L.tile_off;
     const int ox0 = (t % L.tiles_x) * TX, oy0 = (t / L.tiles_x) * TY;
     // the tile's responses inside the map (vw x vh; fewer at the map's right and bottom
     // edges) and the pooled L2 positions they read (qw x qh). Work items are numbered
@@ -2279,15 +2289,15 @@ void launch_l1(const float* gray, int gp, int gw, int gh, const LevelDesc* d_lev
 
 template <int TX, int TY, bool H>
 void launch_resp(const LevelDesc* d_levels, int nlevels, int ntiles, const float* W1, const float* s1buf, float* resp,
-                 float t1, int* meta, Candidate* cands, Candidate* hcands, int hcap, hipStream_t st,
-                 const hipEvent_t* kev, const unsigned char* keep)
+                 float t1, int* meta, Candidate* cands, Candidate* hcands, int hcap, const int2* order,
+                 hipStream_t st, const hipEvent_t* kev, const unsigned char* keep)
 {
     if (kev)
         hipExtLaunchKernelGGL(k_stage1<TX, TY, H>, dim3(ntiles), dim3(S1_THREADS), 0, st, kev[2], kev[3], 0, d_levels,
-                              nlevels, W1, s1buf, resp, t1, meta, cands, hcands, hcap, keep);
+                              nlevels, W1, s1buf, resp, t1, meta, cands, hcands, hcap, order, keep);
     else
         k_stage1<TX, TY, H><<<ntiles, S1_THREADS, 0, st>>>(d_levels, nlevels, W1, s1buf, resp, t1, meta, cands, hcands,
-                                                           hcap, keep);
+                                                           hcap, order, keep);
 }
 
 }  // namespace
@@ -2306,7 +2316,9 @@ hipError_t launch_stage1_t(const float* gray, int gp, int gw, int gh, const Leve
     else if (t.ax == 64 && t.ay == 8) launch_l1<64, 8, H>(L1_ARGS);
     else return hipErrorInvalidValue;
 #undef L1_ARGS
-#define RESP_ARGS d_levels, nlevels, ntiles, W1, s1buf, resp, t1, meta, cands, hcands, hcap, st, kev, keep
+    // l1_order (optional) holds the nl1tiles L1 tiles, then the ntiles stage-1 tiles
+    const int2* s1_order = l1_order ? l1_order + nl1tiles : nullptr;
+#define RESP_ARGS d_levels, nlevels, ntiles, W1, s1buf, resp, t1, meta, cands, hcands, hcap, s1_order, st, kev, keep
     if (t.tx == 16 && t.ty == 16) launch_resp<16, 16, H>(RESP_ARGS);
     else if (t.tx == 32 && t.ty == 8) launch_resp<32, 8, H>(RESP_ARGS);
     else if (t.tx == 16 && t.ty == 8) launch_resp<16, 8, H>(RESP_ARGS);
