@@ -55,6 +55,22 @@ def test_stage1_fp16_error_vs_fp64(api, oracle):
         assert e16 < 0.02
 
 
+def test_stage1_fp16_centring_on_smooth_levels(api, oracle):
+    # the FP16 L1 rounds each 5x5 window less its centre sample: on a flat level the
+    # differences are zero and on a smooth ramp they stay small, so the FP16 stage-1 error
+    # there is far below the 8-bit-scale rounding of raw samples (0.02 on photo levels)
+    from oracle.net64 import forward64, max_abs_error
+    yy, xx = np.mgrid[0:180, 0:240].astype(np.float32)
+    levels = {"flat 255": np.full((180, 240), 255.0, np.float32),
+              "flat 37": np.full((180, 240), 37.0, np.float32),
+              "ramp": (120.0 + 0.5 * xx - 0.25 * yy).astype(np.float32)}
+    for name, lvl in levels.items():
+        e64 = forward64(oracle.models[0], lvl)
+        e16 = max_abs_error(api.stage1(lvl, precision=api.FP16_BF16), e64)
+        print("stage 1 %s level: max |error| of FP16 vs FP64 %.3g" % (name, e16), flush=True)
+        assert e16 < 2e-3, name
+
+
 def test_stage23_bf16_fp8_error_vs_fp64(api, oracle):
     from oracle.net64 import forward64
     # the check inputs of real candidates (faces and background windows) and random ones
