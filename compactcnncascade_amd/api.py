@@ -49,6 +49,11 @@ class CImage(ctypes.Structure):
     _fields_ = [("cols", _I), ("rows", _I), ("channels", _I), ("data", _P), ("step", ctypes.c_size_t)]
 
 
+# ccnn_image as a numpy record (the same 32 bytes as CImage), for building many at once
+CIMAGE_DTYPE = np.dtype({"names": ["cols", "rows", "channels", "data", "step"],
+                         "formats": [np.int32, np.int32, np.int32, np.uint64, np.uint64],
+                         "offsets": [0, 4, 8, 16, 24], "itemsize": 32})
+
 FACE_DTYPE = np.dtype([("x", np.int32), ("y", np.int32), ("width", np.int32), ("height", np.int32),
                        ("score", np.float32), ("gender", np.int32), ("smile", np.int32), ("glasses", np.int32)])
 
@@ -230,7 +235,8 @@ class FaceDetector:
         img, ci = _image(img)
         cap = 4096
         while True:
-            faces = np.zeros(cap, FACE_DTYPE)
+            # the library writes the first min(n, cap) records; only those are returned
+            faces = np.empty(cap, FACE_DTYPE)
             n = self._L.ccnn_detect(self._h, _ptr(faces), cap, ctypes.byref(ci))
             if n <= cap:
                 break
@@ -246,7 +252,7 @@ class FaceDetector:
         ci = CImage(cols, rows, channels, dev_ptr, step)
         cap = 4096
         while True:
-            faces = np.zeros(cap, FACE_DTYPE)
+            faces = np.empty(cap, FACE_DTYPE)
             n = self._L.ccnn_detect_device(self._h, _ptr(faces), cap, ctypes.byref(ci))
             if n <= cap:
                 break
@@ -262,14 +268,25 @@ class FaceDetector:
             cimgs = (CImage * len(keep))(*[ci for _, ci in keep])
             device = 0
         else:
-            cimgs = (CImage * len(device_frames))(*[CImage(c, r, ch, p, st) for (p, c, r, ch, st) in device_frames])
+            try:
+                # integer pointers: the records in one pass
+                a = np.asarray(device_frames, dtype=np.uint64).reshape(len(device_frames), 5)
+                cimgs = np.empty(len(device_frames), CIMAGE_DTYPE)
+                cimgs["data"], cimgs["cols"], cimgs["rows"] = a[:, 0], a[:, 1], a[:, 2]
+                cimgs["channels"], cimgs["step"] = a[:, 3], a[:, 4]
+            except (TypeError, ValueError):
+                cimgs = (CImage * len(device_frames))(*[CImage(c, r, ch, p, st)
+                                                        for (p, c, r, ch, st) in device_frames])
             device = 1
         n = len(cimgs)
-        faces = np.zeros(max(1, n * capacity), FACE_DTYPE)
+        # the library writes each frame's first min(count, capacity) records; only those
+        # are returned
+        faces = np.empty(max(1, n * capacity), FACE_DTYPE)
         counts = np.zeros(max(1, n), np.int32)
-        if self._L.ccnn_detect_batch(self._h, n, cimgs, device, _ptr(faces), capacity, _ptr(counts)) != 0:
+        ip = cimgs.ctypes.data_as(_P) if isinstance(cimgs, np.ndarray) else cimgs
+        if self._L.ccnn_detect_batch(self._h, n, ip, device, _ptr(faces), capacity, _ptr(counts)) != 0:
             raise RuntimeError("ccnn_detect_batch failed")
-        return [(int(counts[i]), faces[i * capacity:i * capacity + min(int(counts[i]), capacity)]) for i in range(n)]
+        return [(c, faces[i * capacity:i * capacity + min(c, capacity)]) for i, c in enumerate(counts[:n].tolist())]
 
     def set_batch_mode(self, mode):
         """1: DetectBatch runs each batch as one graph launch (default); 0: per-frame

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <thread>
+#include <sched.h>
 
 namespace ccnn {
 
@@ -166,6 +167,15 @@ int Detector::detect_batch_graph(int n, const ccnn_image* imgs, bool device_ptr,
         seg.push_back(f);
     }
     return flush();
+}
+
+// CPUs this process may run on (its affinity mask), for the replay pool's size
+static int host_cpus()
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return CPU_COUNT(&set);
+    return (int)std::thread::hardware_concurrency();
 }
 
 int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std::vector<std::vector<ccnn_face>>& faces)
@@ -509,7 +519,7 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
                                               checks, st);
         emit_faces(rects, f.pre, faces[f.idx]);
     };
-    if (!pool_) pool_.reset(new ReplayPool(std::max(0, std::min<int>((int)std::thread::hardware_concurrency(), 8) - 1)));
+    if (!pool_) pool_.reset(new ReplayPool(std::max(0, std::min(host_cpus(), 16) - 1)));
     const std::function<void(int)> one = [&](int fi) {
         thread_local std::vector<Candidate> lc;
         thread_local std::vector<CheckResult> lr;
