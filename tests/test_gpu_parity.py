@@ -429,6 +429,54 @@ def test_batch_matches_single_frame_results(api, golden):
         assert cb == cs and np.array_equal(fb, fs)
 
 
+def test_device_frames_match_host_frames(api):
+    # frames already in GPU memory (ccnn_detect_device, ccnn_detect_batch with device = 1)
+    # give the same results as the same frames from host memory: BGR, BGRA and gray
+    # frames, a blank frame (no candidates), one below the 47x51 minimum (no faces) and a
+    # device view whose row step exceeds cols * channels; one Detect at a time, and whole
+    # batches in both batch modes; empty batches return nothing
+    import torch
+    from cases import vga
+    rng = np.random.default_rng(11)
+    f0, f1 = vga(0), vga(4)
+    frames = [f0, np.ascontiguousarray(np.concatenate([f1, rng.integers(0, 256, f1.shape[:2] + (1,),
+                                                                        dtype=np.uint8)], axis=2)),
+              np.ascontiguousarray(f0[:, :, 1]), np.zeros((300, 400, 3), np.uint8),
+              np.ascontiguousarray(f1[:40, :44]), vga(2)]
+    dev = [torch.from_numpy(f).to("cuda:0") for f in frames]
+    descs = [(t.data_ptr(), f.shape[1], f.shape[0], 1 if f.ndim == 2 else f.shape[2], f.strides[0])
+             for t, f in zip(dev, frames)]
+    # a device view: the last frame inside a wider buffer, rows 2,048 bytes apart
+    f = frames[-1]
+    wide = torch.zeros((f.shape[0], 2048), dtype=torch.uint8, device="cuda:0")
+    wide[:, :f.shape[1] * 3] = torch.from_numpy(np.ascontiguousarray(f).reshape(f.shape[0], -1)).to("cuda:0")
+    frames.append(f)
+    descs.append((wide.data_ptr(), f.shape[1], f.shape[0], 3, 2048))
+    torch.cuda.synchronize()
+
+    def fresh():
+        d = api.FaceDetector()
+        assert d.Init(api_param(frames[0], dict(min_face=40, max_face=200))) == 0
+        return d
+
+    d_host, d_dev = fresh(), fresh()
+    host = [d_host.Detect(f) for f in frames]
+    for (p, w, h, c, st), (nh, fh) in zip(descs, host):
+        nd, fd = d_dev.DetectDevice(p, w, h, c, st)
+        assert nd == nh and np.array_equal(fd, fh), (w, h, c)
+    assert host[3][0] == 0 and host[4][0] == 0 and host[0][0] > 0
+    for mode in (1, 0):
+        d_host, d_dev = fresh(), fresh()
+        d_host.set_batch_mode(mode)
+        d_dev.set_batch_mode(mode)
+        bh = d_host.DetectBatch(images=frames)
+        bd = d_dev.DetectBatch(device_frames=descs)
+        assert len(bh) == len(bd) == len(frames)
+        for (nh, fh), (nd, fd) in zip(bh, bd):
+            assert nd == nh and np.array_equal(fd, fh), mode
+        assert d_host.DetectBatch(images=[]) == [] and d_dev.DetectBatch(device_frames=[]) == []
+
+
 def test_gray_image_of_last_frame(api, oracle):
     # getGrayImage32F (CompactCNNLibAPI_v2.cpp:462-507): the FP32 gray image the last
     # Detect worked on, as a host copy (Pipeline::CPU) and as a device pointer
