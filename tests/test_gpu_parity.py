@@ -477,6 +477,72 @@ def test_device_frames_match_host_frames(api):
         assert d_host.DetectBatch(images=[]) == [] and d_dev.DetectBatch(device_frames=[]) == []
 
 
+def test_parameter_accessors_and_profiling(api):
+    # FaceDetector's getters and setters (CompactCNNLib.h:172-200): each setter gives the
+    # detections of a detector initialised with the changed parameter; profiling
+    # (per-kernel HIP events, the batch then runs without its graph) changes no result
+    from cases import vga
+    frames = [vga(0), vga(3), vga(5)]
+
+    def fresh(**kw):
+        d = api.FaceDetector()
+        assert d.Init(api_param(frames[0], dict(dict(min_face=40, max_face=200), **kw))) == 0
+        return d
+
+    def same(a, b):
+        return a[0] == b[0] and np.array_equal(a[1], b[1])
+
+    d = fresh()
+    p = d.getParam()
+    assert p.max_image_size == (640, 480) and p.min_face_height == 40 and p.max_face_height == 200
+    assert d.getMaxImageSize() == (640, 480) and d.getMaxObjectHeight() == 200
+    # setParam re-initialises with the new parameters
+    p.min_face_height = 60
+    d.setParam(p)
+    q = api.FaceDetector()
+    assert q.Init(p) == 0
+    assert same(d.Detect(frames[1]), q.Detect(frames[1]))
+    # max object height and max image size
+    d = fresh()
+    d.setMaxObjectHeight(120)
+    assert d.getMaxObjectHeight() == 120
+    q = fresh(max_face=120)
+    assert all(same(d.Detect(f), q.Detect(f)) for f in frames)
+    d = fresh()
+    d.setMaxImageSize((800, 600))
+    assert d.getMaxImageSize() == (800, 600)
+    pq = api_param(frames[0], dict(min_face=40, max_face=200))
+    pq.max_image_size = (800, 600)
+    q = api.FaceDetector()
+    assert q.Init(pq) == 0
+    assert all(same(d.Detect(f), q.Detect(f)) for f in frames)
+    # the thread count is recorded (the GPU path has no host worker threads to size)
+    d.setNumThreads(3)
+    assert d.getNumThreads() == 3
+    d.set_compute_precision(api.FP32_BF16)
+    assert d.get_compute_precision() == api.FP32_BF16
+    # profiling on: per-frame kernel times, and the batch's per-group times
+    plain = fresh()
+    base = [plain.Detect(f) for f in frames]
+    batch = fresh().DetectBatch(images=frames)
+    prof = fresh()
+    prof.set_profiling(True)
+    got = [prof.Detect(f) for f in frames]
+    t = prof.last_timing()
+    assert all(same(a, b) for a, b in zip(got, base))
+    assert t["frames"] == len(frames)
+    assert all(t["last"][k] > 0 for k in ("gray", "stage1_l1", "stage1", "check", "total"))
+    assert t["totals"]["total"] >= t["last"]["total"]
+    prof = fresh()
+    prof.set_profiling(True)
+    got = prof.DetectBatch(images=frames)
+    bt = prof.last_batch_timing()
+    assert all(same(a, b) for a, b in zip(got, batch))
+    assert bt["frames"] == len(frames)
+    assert all(bt[k] > 0 for k in ("gray", "stage1_l1", "stage1", "compact", "check", "total"))
+    assert bt["total"] >= bt["stage1"] + bt["check"]
+
+
 def test_gray_image_of_last_frame(api, oracle):
     # getGrayImage32F (CompactCNNLibAPI_v2.cpp:462-507): the FP32 gray image the last
     # Detect worked on, as a host copy (Pipeline::CPU) and as a device pointer
