@@ -241,13 +241,17 @@ __global__ void __launch_bounds__(256) k_gray(const uint8_t* __restrict__ img, i
 __global__ void __launch_bounds__(256) k_gray_batch(const FrameDesc* __restrict__ frames, int nframes,
                                                     int total_quads, float* __restrict__ gray)
 {
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= total_quads) return;
+    // the frame of the block's first group, by a search on block-uniform values (scalar
+    // loads), then each thread steps to a later frame when the block straddles frames
+    const int q0 = blockIdx.x * 256;
     int lo = 0, hi = nframes - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (frames[mid].quads_off <= q) lo = mid; else hi = mid - 1;
+        if (frames[mid].quads_off <= q0) lo = mid; else hi = mid - 1;
     }
+    const int q = q0 + threadIdx.x;
+    if (q >= total_quads) return;
+    while (lo + 1 < nframes && frames[lo + 1].quads_off <= q) ++lo;
     const FrameDesc& F = frames[lo];
     const int qq = q - F.quads_off;
     const int y = qq / F.qw;
