@@ -234,15 +234,24 @@ def test_random_frames_and_parameters_match_oracle(api, oracle, seed):
     assert bits(faces["score"]).tolist() == bits(np.asarray(exp[:, 4], np.float32)).tolist()
     assert [[int(r["gender"]), int(r["smile"]), int(r["glasses"])] for r in faces] == \
         [[int(v) for v in e[5:8]] for e in exp]
-    # the same frame twice through DetectBatch, as one graph launch and pipelined per frame
-    for mode in (1, 0):
-        d = api.FaceDetector()
-        assert d.Init(api_param(f, over)) == 0
-        d.set_batch_mode(mode)
-        for nb, fb in d.DetectBatch(images=[f, f], capacity=4096):
-            # the count is always reported; the records up to the capacity
-            assert nb == n and len(fb) == min(n, 4096), (seed, mode)
-            assert np.array_equal(fb, faces[:len(fb)]), (seed, mode)
+    # the same frame twice through DetectBatch, as one graph launch and pipelined per frame;
+    # in a reduced compute precision too, against Detect in that precision (the same
+    # kernels on both paths)
+    for prec in (api.FP32, api.FP16_BF16):
+        if prec != api.FP32:
+            d = api.FaceDetector()
+            assert d.Init(api_param(f, over)) == 0
+            d.set_compute_precision(prec)
+            n, faces = d.Detect(f)
+        for mode in (1, 0):
+            d = api.FaceDetector()
+            assert d.Init(api_param(f, over)) == 0
+            d.set_compute_precision(prec)
+            d.set_batch_mode(mode)
+            for nb, fb in d.DetectBatch(images=[f, f], capacity=4096):
+                # the count is always reported; the records up to the capacity
+                assert nb == n and len(fb) == min(n, 4096), (seed, prec, mode)
+                assert np.array_equal(fb, faces[:len(fb)]), (seed, prec, mode)
 
 
 def test_trace_matches_oracle(api, oracle):
