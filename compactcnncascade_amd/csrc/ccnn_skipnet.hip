@@ -2,7 +2,7 @@
 // gray frame, which stage-1 tiles of which pyramid levels can hold a window that the
 // cascade accepts, so that the others are not computed. No counterpart in the reference:
 // the plain cascade is its fallback (tiles the network is not confident about run as
-// usual) and the judge of its recall.
+// usual) and the reference its recall is measured against.
 //
 // Network (SkipNet): input x0 = mean of each 8x8 block of the FP32 gray image / 255 - 0.5
 // (H8 x W8 = ceil(H/8) x ceil(W/8)); three 3x3 convolutions with zero padding ("same"),
