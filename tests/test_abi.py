@@ -125,3 +125,22 @@ int main(int argc, char** argv) {
            "-L" + os.path.dirname(LIB), "-Wl,-rpath," + os.path.dirname(LIB), "-lccnn"]
     subprocess.run(cmd, check=True)
     assert exe.exists()
+
+
+def test_image_records_match_the_c_struct():
+    # DetectBatch builds device-frame descriptors as numpy records; they must lay out
+    # exactly as ccnn_image (cols, rows, channels, 4 bytes of padding, data, step)
+    import numpy as np
+    from compactcnncascade_amd import api
+    assert api.CIMAGE_DTYPE.itemsize == ctypes.sizeof(api.CImage) == 32
+    for name in ("cols", "rows", "channels", "data", "step"):
+        assert api.CIMAGE_DTYPE.fields[name][1] == getattr(api.CImage, name).offset, name
+    frames = [(0x7f0012340000 + 977 * i, 640 + i, 480 - i, (1, 3, 4)[i % 3], 3 * (640 + i) + 64) for i in range(9)]
+    rec = np.empty(len(frames), api.CIMAGE_DTYPE)
+    a = np.asarray(frames, dtype=np.uint64)
+    rec["data"], rec["cols"], rec["rows"], rec["channels"], rec["step"] = a[:, 0], a[:, 1], a[:, 2], a[:, 3], a[:, 4]
+    ref = bytes((api.CImage * len(frames))(*[api.CImage(c, r, ch, p, st) for (p, c, r, ch, st) in frames]))
+    got = rec.tobytes()
+    for i in range(len(frames)):
+        assert got[32 * i:32 * i + 12] == ref[32 * i:32 * i + 12]
+        assert got[32 * i + 16:32 * i + 32] == ref[32 * i + 16:32 * i + 32]
