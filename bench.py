@@ -20,6 +20,7 @@ one process per core). Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--frames F] [--gpus 1] [--facial]
                     [--no-secondary] [--no-cpu-baseline] [--profile-pass N]
+                    [--phase-pass check|stage1]
 """
 import argparse
 import ctypes
@@ -411,6 +412,83 @@ def profile_pass(n, mode, workload="fhd_min40_max240", precision=0):
     print(json.dumps({"profile_pass": n, "mode": mode, "workload": workload, "precision": precision}), flush=True)
 
 
+def phase_pass(kind, workload="fhd_min40_max240"):
+    """Per-phase latency from the kernels' phase timestamps (the profiling build
+    libccnn_phases.so): 'check' = the k_check items of 8 single 1080p frames (one Detect
+    each, after 16 untimed ones), 'stage1' = the k_stage1 tiles of one 64-frame batch (the
+    graph path, after 3 untimed batches). Prints one JSON line of per-phase microseconds."""
+    import torch
+    torch.zeros(1, device="cuda:0")
+    from compactcnncascade_amd import api
+    api.LIB_PATH = api.PHASES_LIB_PATH
+    if not os.path.exists(api.LIB_PATH):
+        raise SystemExit("libccnn_phases.so is not built: python -m compactcnncascade_amd.build --phase-stamps")
+    cfg = synth.WORKLOADS[workload]
+    det = detector(api, cfg["w"], cfg["h"], cfg["min_face"], cfg["max_face"], cfg["scale_factor"])
+    us = lambda a: np.round(a / 100.0, 3)              # 100 MHz ticks -> microseconds
+
+    def stats(v):
+        v = np.asarray(v, np.float64)
+        return {"mean": round(float(v.mean()), 2), "median": round(float(np.median(v)), 2),
+                "max": round(float(v.max()), 2)} if len(v) else None
+
+    if kind == "check":
+        frames = make_frames(workload, 24)
+        dev, df = to_device(torch, frames)
+        for (p, w, h, c, s) in df[:16]:
+            det.DetectDevice(ctypes.c_void_p(p), w, h, c, s)
+        torch.cuda.synchronize()
+        names2 = ["crop", "stage-2 input", "stage-2 L1", "L2", "sums", "L3", "hidden", "output"]
+        names3 = ["stage-3 input + L1", "L2", "sums", "L3", "hidden", "output", "to item end"]
+        per2 = {n: [] for n in names2}
+        per3 = {n: [] for n in names3}
+        tot2, tot3, spans, items, s3n = [], [], [], [], []
+        api.phase_stamps(0, 0)
+        for (p, w, h, c, s) in df[16:24]:
+            det.DetectDevice(ctypes.c_void_p(p), w, h, c, s)
+            torch.cuda.synchronize()
+            a = api.phase_stamps(0, 1024).astype(np.int64)
+            a = a[(a[:, 0] > 0) & (a[:, 15] > 0)]
+            t = a[:, :17] - a[:, 0].min()
+            s3 = a[:, 22] == 1
+            for k, n in enumerate(names2):
+                per2[n] += list(us(t[:, k + 1] - t[:, k]))
+            seq3 = [8, 9, 10, 11, 12, 13, 14, 15]
+            for k, n in enumerate(names3):
+                per3[n] += list(us(t[s3, seq3[k + 1]] - t[s3, seq3[k]]))
+            tot2 += list(us(t[~s3, 15] - t[~s3, 0]))
+            tot3 += list(us(t[s3, 15] - t[s3, 0]))
+            spans.append(float(us(t[:, 15].max())))
+            items.append(len(a))
+            s3n.append(int(s3.sum()))
+        out = {"phase_pass": "check", "workload": workload, "frames": 8, "items_per_frame": items,
+               "stage3_items_per_frame": s3n, "stage2_phases_us": {n: stats(v) for n, v in per2.items()},
+               "stage3_phases_us": {n: stats(v) for n, v in per3.items()},
+               "item_us": {"stage2_only": stats(tot2), "with_stage3": stats(tot3)},
+               "span_us": stats(spans)}
+    else:
+        frames = make_frames(workload, 64)
+        dev, df = to_device(torch, frames)
+        for _ in range(3):
+            det.DetectBatch(device_frames=df)
+        torch.cuda.synchronize()
+        api.phase_stamps(1, 0)
+        det.DetectBatch(device_frames=df)
+        torch.cuda.synchronize()
+        a = api.phase_stamps(1, 65536).astype(np.int64)
+        a = a[(a[:, 0] > 0) & (a[:, 4] > 0)]
+        t = a[:, :5] - a[:, 0].min()
+        names = ["tile lookup + L1-sum window to LDS", "L2", "L2 sums + tail weights to LDS",
+                 "L3 + hidden + output (thread 0)"]
+        cus = len(np.unique(a[:, 20]))
+        life = us(t[:, 4] - t[:, 0])
+        out = {"phase_pass": "stage1", "workload": workload, "frames": 64, "tiles": int(len(a)),
+               "phases_us": {n: stats(us(t[:, k + 1] - t[:, k])) for k, n in enumerate(names)},
+               "tile_us": stats(life), "span_us": round(float(us(t[:, 4].max())), 1),
+               "mean_tiles_in_flight_per_cu": round(float(life.sum() / (float(us(t[:, 4].max())) * cus)), 2)}
+    print(json.dumps(out), flush=True)
+
+
 def autotune_report(steps):
     """The batch path's autotuner (ccnn_autotune: every stage-1 tile shape, then 2/3/4
     persistent check workgroups per CU, each timed as whole DetectBatch passes) on the
@@ -463,6 +541,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="only run the batch path's autotuner on the benchmark workloads and exit")
     ap.add_argument("--profile-precision", type=int, default=0, choices=[0, 1, 2, 3, 4])
+    ap.add_argument("--phase-pass", choices=["check", "stage1"], default=None,
+                    help="per-phase latency of k_check items (8 single frames) or k_stage1 tiles (one "
+                         "64-frame batch) from the phase-stamp build (build.py --phase-stamps)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("DB", "CALIB_DB", "COUNTER", "FRAMES"),
                     help="summarize a rocprofv3 counter database per kernel and exit")
     args = ap.parse_args()
@@ -483,6 +564,9 @@ def main():
         return
     if args.profile_pass:
         profile_pass(args.profile_pass, args.profile_mode, args.workload, args.profile_precision)
+        return
+    if args.phase_pass:
+        phase_pass(args.phase_pass, args.workload)
         return
 
     import torch

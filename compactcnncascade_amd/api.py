@@ -13,6 +13,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libccnn.so")
+# the profiling build with the kernels' phase timestamps (build.py --phase-stamps); load()
+# takes it when LIB_PATH is set to it before the first load
+PHASES_LIB_PATH = os.path.join(_HERE, "libccnn_phases.so")
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -75,6 +78,7 @@ def load():
     L.ccnn_detect.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
     L.ccnn_detect_device.argtypes = [_P, _P, _I, ctypes.POINTER(CImage)]
     L.ccnn_detect_batch.argtypes = [_P, _I, _P, _I, _P, _I, _P]
+    L.ccnn_phase_stamps.argtypes = [_I, _P, _I]
     L.ccnn_clear.argtypes = [_P]
     L.ccnn_is_empty.argtypes = [_P]
     L.ccnn_get_param.argtypes = [_P, ctypes.POINTER(CParam)]
@@ -136,6 +140,19 @@ def _image(img):
         img = np.ascontiguousarray(img, np.uint8)
     h, w = img.shape[:2]
     return img, CImage(w, h, ch, _ptr(img), img.strides[0])
+
+
+PHASE_SLOTS = 24
+
+
+def phase_stamps(kernel, nblocks):
+    """The phase timestamps of kernel 0 (k_check) or 1 (k_stage1) for the first nblocks
+    workgroups, uint64 [nblocks, 24] (100 MHz clock; 0 where not reached), then cleared;
+    None when the loaded library was built without them (ccnn_phase_stamps)."""
+    out = np.zeros((nblocks, PHASE_SLOTS), np.uint64)
+    if load().ccnn_phase_stamps(int(kernel), _ptr(out), int(nblocks)) != 0:
+        return None
+    return out
 
 
 class Param:

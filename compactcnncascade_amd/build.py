@@ -1,6 +1,7 @@
 """Builds libccnn.so (HIP kernels for gfx950 + C++ host side + C ABI) in-tree.
 
     python -m compactcnncascade_amd.build
+    python -m compactcnncascade_amd.build --phase-stamps    # libccnn_phases.so (profiling)
 
 The shipped model files are embedded into the library as byte arrays, as the
 reference embeds them (serialized_models.h), so the library has no run-time file
@@ -15,6 +16,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libccnn.so")
+# the profiling build with the check and stage-1 kernels' phase timestamps compiled in
+PHASES_LIB = os.path.join(PKG, "libccnn_phases.so")
 MODELS = ["cnn4face1_cntk.bin", "cnn4face2_cntk.bin", "cnn4face3_cntk.bin", "cnn4landmarks_cntk.bin"]
 
 SOURCES = ["ccnn_kernels.hip", "ccnn_noalloc.hip", "ccnn_skipnet.hip", "ccnn_detector.cpp", "ccnn_batch.cpp",
@@ -45,12 +48,15 @@ def gen_models(path):
             f.write(text)
 
 
-def build(verbose=True):
+def build(verbose=True, phase_stamps=False):
+    """Compiles libccnn.so for gfx950 (with phase_stamps: libccnn_phases.so, the same
+    library with the kernels' phase timestamps, for ccnn_phase_stamps)."""
     os.makedirs(BUILD, exist_ok=True)
     models_cpp = os.path.join(BUILD, "ccnn_models.cpp")
     gen_models(models_cpp)
     srcs = [os.path.join(CSRC, s) for s in SOURCES] + [models_cpp]
-    tmp = LIB + ".tmp"
+    lib = PHASES_LIB if phase_stamps else LIB
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
            # exact path: no a*b+c contraction anywhere (the reference rounds each op)
            "-ffp-contract=off",
@@ -60,7 +66,7 @@ def build(verbose=True):
            # every symbol the library references must resolve at link time
            "-Wl,-z,defs",
            "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-           "-o", tmp] + srcs
+           "-o", tmp] + (["-DCCNN_PHASE_STAMPS"] if phase_stamps else []) + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -69,10 +75,10 @@ def build(verbose=True):
         print(out)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed (%d):\n%s" % (r.returncode, out))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    build()
+    build(phase_stamps="--phase-stamps" in sys.argv[1:])
     sys.exit(0)

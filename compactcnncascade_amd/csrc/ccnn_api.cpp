@@ -137,6 +137,11 @@ int ccnn_set_skipnet(ccnn_detector* d, const float* params, float threshold, int
     return d->det.set_skipnet(params, threshold, min_face, scale_factor);
 }
 
+int ccnn_phase_stamps(int kernel, unsigned long long* out, int nblocks)
+{
+    return ccnn::read_phase_stamps(kernel, out, nblocks);
+}
+
 int ccnn_last_batch_timing(const ccnn_detector* d, float* ms)
 {
     if (!d || !ms) return -1;

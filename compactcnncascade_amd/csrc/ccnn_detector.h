@@ -35,6 +35,8 @@ hipError_t launch_stage1(const float* gray, int gp, int gw, int gh, const LevelD
 size_t level_l1_layout(LevelDesc& L, int shape = 0);
 // cap: candidates stored in cands (the list may count more, then the caller redoes the
 // work); queue (optional): a zeroed counter from which a persistent grid takes its items
+// phase timestamps of the profiling build (ccnn_kernels.hip)
+int read_phase_stamps(int kern, unsigned long long* out, int nblocks);
 hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDesc* d_levels, const Candidate* cands,
                         const int* meta, const CheckParams& P, const float* W2, const float* W3, int2* s2res,
                         int2* s3res, CheckResult* results, int nblocks, int* hmeta, Candidate* hcands,

@@ -29,7 +29,59 @@
 #include <hip/hip_ext.h>
 #include "ccnn_internal.h"
 
+#include <type_traits>
+
 namespace ccnn {
+
+// Phase timestamps, compiled in only by profiling builds (CCNN_PHASE_STAMPS): thread 0
+// of each of the first PHASE_BLOCKS workgroups of k_check and of k_stage1 records the
+// 100 MHz wall clock (s_memrealtime) at fixed points of its work; ccnn_phase_stamps reads
+// them. Slots, k_check (per workgroup, its last item): 0 item start, 1 crop prepared,
+// 2 stage-2 input written, 3-8 after each barrier of the stage-2 network (L1, L2, sums,
+// L3, hidden layer, output layer), 9-14 the same for stage 3, 15 item end, 16 stage-3
+// input written; 20 hardware id, 21 item, 22 stage 3 ran. k_stage1 (per tile): 0 start,
+// 1 L1-sum window in LDS, 2 L2 done, 3 L2 sums and tail weights in LDS, 4 response
+// written (thread 0's), 20 hardware id.
+// The stamps collect in LDS (so that no global address stays live in registers) and go
+// to memory once, at the end of the workgroup's work (PHASE_FLUSH).
+constexpr int PHASE_BLOCKS = 65536, PHASE_SLOTS = 24;
+#ifdef CCNN_PHASE_STAMPS
+__device__ unsigned long long g_phase[2][PHASE_BLOCKS][PHASE_SLOTS];
+__shared__ unsigned long long s_phase[PHASE_SLOTS];
+#define PHASE_SET(kern, slot, v)                   \
+    do {                                           \
+        if (threadIdx.x == 0) s_phase[slot] = (v); \
+    } while (0)
+#define PHASE_CLEAR()                                                       \
+    do {                                                                    \
+        if (threadIdx.x == 0)                                               \
+            for (int i_ = 0; i_ < PHASE_SLOTS; ++i_) s_phase[i_] = 0;        \
+    } while (0)
+#define PHASE_FLUSH(kern)                                                                       \
+    do {                                                                                        \
+        if (threadIdx.x == 0 && blockIdx.x < PHASE_BLOCKS)                                      \
+            for (int i_ = 0; i_ < PHASE_SLOTS; ++i_) g_phase[kern][blockIdx.x][i_] = s_phase[i_]; \
+    } while (0)
+#else
+#define PHASE_SET(kern, slot, v) \
+    do {                         \
+    } while (0)
+#define PHASE_CLEAR() \
+    do {              \
+    } while (0)
+#define PHASE_FLUSH(kern) \
+    do {                  \
+    } while (0)
+#endif
+#define PHASE(kern, slot) PHASE_SET(kern, slot, wall_clock64())
+// the stamps cost k_stage1 registers; the profiling build keeps its occupancy at the
+// default build's 5 waves per SIMD so that the stamped phases are those of the product
+#ifdef CCNN_PHASE_STAMPS
+#define S1_WAVES __attribute__((amdgpu_waves_per_eu(5)))
+#else
+#define S1_WAVES
+#endif
+enum { PH_CHECK = 0, PH_STAGE1 = 1 };
 
 // The reciprocal of the rational tanh's denominator d >= 1, as 1 - 1/d needs it: 1/d
 // rounded to nearest for 1 <= d < 2^25, from the hardware reciprocal estimate (within
@@ -596,7 +648,7 @@ __global__ void __launch_bounds__(S1A_THREADS) k_stage1_l1(const float* __restri
 }
 
 template <int TX, int TY, bool H>
-__global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
+__global__ void __launch_bounds__(S1_THREADS) S1_WAVES k_stage1(const LevelDesc* __restrict__ levels, int nlevels,
                                                        const float* __restrict__ W,
                                                        const float* __restrict__ s1buf, float* __restrict__ resp,
                                                        float t1, int* __restrict__ meta, Candidate* __restrict__ cands,
@@ -619,6 +671,9 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
 #pragma unroll
     for (int k = 0; k < WPER; ++k) wt[k] = W[N::oK3 + min(tid + k * S1_THREADS, WT - 1)];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    PHASE_CLEAR();
+    PHASE(PH_STAGE1, 0);
+    PHASE_SET(PH_STAGE1, 20, __smid());
     // order (optional): each tile's {level, tile within the level}, one load instead of
     // the binary search over the level table (11 dependent loads for a 64-frame batch)
     int li, t;
@@ -674,6 +729,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         }
     }
     __syncthreads();
+    PHASE(PH_STAGE1, 1);
 
     // L2, one pooled L2 position per thread: the eight maps (two conv_3x3 chains per L1
     // sum + lrelu_bn_max), then their neighbour sums, held in registers until every thread
@@ -748,6 +804,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         sv[k][N::M2 - 1] = v[N::M2 - 2] + v[N::M2 - 1];
     }
     __syncthreads();
+    PHASE(PH_STAGE1, 2);
     // the L2 sums over the (consumed) L1 sums, with rows P2S floats apart: an odd multiple
     // of 8, so that the rows of an 8 x 8 block of responses (below) start in distinct
     // groups of 8 LDS banks and the L3 window reads of a wave are free of bank conflicts;
@@ -776,6 +833,7 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
         for (int k = tid; k < Stage1H::SIZE - Stage1H::oL3; k += S1_THREADS) s_wh[k] = src[k];
     }
     __syncthreads();
+    PHASE(PH_STAGE1, 3);
 
     // L3 (conv_5x4: row sums accumulated, cnnpp_cplusplus.cpp:158-187) + lrelu_bn,
     // then the hidden and output layers in registers, one response value per thread. In a
@@ -894,6 +952,8 @@ __global__ void __launch_bounds__(S1_THREADS) k_stage1(const LevelDesc* __restri
     const float out = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     const int e = (oy0 + oy) * L.rw + ox0 + ox;
     resp[L.resp_off + e] = out;
+    PHASE(PH_STAGE1, 4);
+    PHASE_FLUSH(PH_STAGE1);
     // candidate windows (score > t1) go to the frame's list in no particular order; the
     // host replays them in the reference's order (level, then raster). Without meta (the
     // batch path) the response maps alone are written and the ordered compaction
@@ -1309,6 +1369,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
                                               const float* __restrict__ mw = nullptr, int* s_red = nullptr)
 {
     using namespace chk;
+    constexpr int PB = std::is_same<N, Stage3>::value ? 9 : 3;   // first phase slot
     // Wl: the network's weights, staged in LDS; read the same for every thread in L1,
     // per thread from L2 on
     const unsigned ut = (unsigned)tid;
@@ -1375,6 +1436,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         }
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 0);
     // L2
     for (unsigned it = ut; it < N::M1 * P2N; it += CHECK_THREADS) {
         const unsigned i = it / P2N, p = it - i * P2N;
@@ -1402,6 +1464,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         }
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 1);
     }
     float* s2 = s.s2();
     for (unsigned p = ut; p < P2N; p += CHECK_THREADS) {
@@ -1414,6 +1477,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         s2[(N::M2 - 1) * P2N + p] = v[N::M2 - 2] + v[N::M2 - 1];
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 2);
     // L3: conv_8x7 single chain (cnnpp_cplusplus.cpp:292-321) + lrelu_bn
     for (unsigned it = ut; it < N::M2 * RN; it += CHECK_THREADS) {
         const unsigned i = it / RN, p = it - i * RN;
@@ -1431,6 +1495,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         }
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 3);
     // hidden layer, each unit's value already multiplied by its output weight: unit
     // u = HLS * gi + tt of response p goes to s.h()[p * HLP + u], so that the output
     // layer's serial sum over u reads its terms with 16-byte loads issued together
@@ -1449,6 +1514,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         s.h()[p * HLP + u] = hv * Wl[N::oOW + tt * N::HLG + gi];
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 4);
     // output layer: the products in unit order, summed serially (the first term alone,
     // then term + sum: IEEE addition commutes, so the order of the sum is what counts)
     if (tid < RN) {
@@ -1469,6 +1535,7 @@ __device__ __forceinline__ void check_forward(const float* __restrict__ in, ChkS
         s.out()[tid] = (af == 0.f) ? (c + 1.f) * 0.5f : af * c;
     }
     __syncthreads();
+    PHASE(PH_CHECK, PB + 5);
 }
 
 // Copies n floats of weights into LDS with all of a thread's loads in flight together
@@ -1696,6 +1763,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         item = s_item;
     }
     bool staged = false;
+    PHASE_CLEAR();
     while (item < items) {
         const int ci = nv == 2 ? item >> 1 : item;
         const int mod = nv == 2 ? item & 1 : 0;
@@ -1705,6 +1773,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const size_t slot = (size_t)ci * 2 + mod;
+        PHASE(PH_CHECK, 0);
         const Candidate cd = cands[ci];
         const LevelDesc& Lv = levels[cd.level];
         const float scale = Lv.scale;
@@ -1718,7 +1787,12 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
         bool remap;
         prepare_crop(g, gp, cr, P.equalize != 0, s, s_w, u, remap, tid, W2, W3, !staged);
         staged = true;
+        PHASE(PH_CHECK, 1);
         write_input(s, u, remap, false, P.reflection != 0, tid);
+        PHASE(PH_CHECK, 2);
+        PHASE_SET(PH_CHECK, 20, __smid());
+        PHASE_SET(PH_CHECK, 21, item);
+        PHASE_SET(PH_CHECK, 22, 0);
         check_forward<Stage2, RP>(s.in, s, s.w, tid, MW2, s_w);
         if (tid == 0) {
             float d[RN];
@@ -1750,6 +1824,8 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
             }
         } else {
         write_input(s, u, false, true, P.reflection != 0, tid);
+        PHASE(PH_CHECK, 16);
+        PHASE_SET(PH_CHECK, 22, 1);
         check_forward<Stage3, RP>(s.in, s, s.w3, tid, MW3, s_w);
         if (tid == 0) {
             float d[RN];
@@ -1769,6 +1845,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
             if (hs3 && ci < hcap) hs3[slot] = make_int2(k2, __float_as_int(m2));
         }
         }
+        PHASE(PH_CHECK, 15);
         // the next item: every thread reads it after the barrier, and the previous item's
         // last LDS reads are done before it is written
         if (queue) {
@@ -1782,6 +1859,7 @@ __global__ void __launch_bounds__(CHECK_THREADS, 4) k_check(const float* __restr
             item = base < items ? stride_item() : items;
         }
     }
+    PHASE_FLUSH(PH_CHECK);
 }
 
 // Decision of CPUCheckDetect (cnn_detector_v3.cpp:1709-1725, 1808-1811) from the stage-2/3
@@ -2460,6 +2538,30 @@ hipError_t launch_check(const float* gray, int gp, int gw, int gh, const LevelDe
         k_check_decide<<<(nblocks + 255) / 256, 256, 0, st>>>(d_levels, cands, meta, P, s2res, s3res, results, hmeta,
                                                               hcands, hresults, hcap);
     return hipGetLastError();
+}
+
+// The phase timestamps of kernel kern (PH_CHECK or PH_STAGE1): nblocks workgroups x
+// PHASE_SLOTS values to out (when not null), then all of that kernel's are cleared. For
+// an idle device; -1 without CCNN_PHASE_STAMPS or for arguments out of range.
+int read_phase_stamps(int kern, unsigned long long* out, int nblocks)
+{
+#ifdef CCNN_PHASE_STAMPS
+    if (kern < 0 || kern > 1 || nblocks < 0 || nblocks > PHASE_BLOCKS) return -1;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(g_phase)) != hipSuccess) return -1;
+    unsigned long long* base = static_cast<unsigned long long*>(p) + (size_t)kern * PHASE_BLOCKS * PHASE_SLOTS;
+    if (out && nblocks > 0 &&
+        hipMemcpy(out, base, sizeof(unsigned long long) * PHASE_SLOTS * (size_t)nblocks, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+        return -1;
+    if (hipMemset(base, 0, sizeof(unsigned long long) * PHASE_SLOTS * (size_t)PHASE_BLOCKS) != hipSuccess) return -1;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+#else
+    (void)kern;
+    (void)out;
+    (void)nblocks;
+    return -1;
+#endif
 }
 
 }  // namespace ccnn

@@ -332,6 +332,18 @@ void ccnn_set_profiling(ccnn_detector* d, int on);
  * proposal network and tile masks, k_stage1_l1, k_stage1, the compaction, k_check, the
  * whole sequence, and the batch's frame count. 0, or -1. */
 int ccnn_last_batch_timing(const ccnn_detector* d, float* ms);
+/* Phase timestamps of the check and stage-1 kernels (no counterpart in the reference; for
+ * profiling builds, compiled with CCNN_PHASE_STAMPS: python -m compactcnncascade_amd.build
+ * --phase-stamps writes libccnn_phases.so). Copies nblocks x 24 values per workgroup of
+ * kernel 0 (k_check) or 1 (k_stage1) into out (NULL: none), then clears them; call with the
+ * device idle. Values are 100 MHz clock stamps by the workgroup's thread 0, 0 where not
+ * reached. k_check (its last item): 0 item start, 1 crop prepared, 2 stage-2 input
+ * written, 3-8 after the stage-2 network's L1, L2, sums, L3, hidden and output layers,
+ * 9-14 the same for stage 3, 15 item end, 16 stage-3 input written; 20 hardware id, 21
+ * item, 22 stage 3 ran. k_stage1 (per tile): 0 start, 1 L1-sum window in LDS, 2 L2 done,
+ * 3 L2 sums in LDS, 4 thread 0's response written; 20 hardware id. 0, or -1 for a library
+ * built without the stamps or arguments out of range. */
+int ccnn_phase_stamps(int kernel, unsigned long long* out, int nblocks);
 int ccnn_last_timing(const ccnn_detector* d, float* ms);
 
 #ifdef __cplusplus
