@@ -188,8 +188,6 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     // layout of the batch: each frame's gray image, level table, L1-sum planes, response
     // maps and stage-1 tiles follow the previous frame's
     std::vector<LevelDesc> lv;
-    // tile tables: every frame's L1 tiles, then every frame's stage-1 tiles (s1order)
-    std::vector<int2> order, s1order;
     std::vector<int> lv_begin(nf + 1, 0), gpitch(nf);
     std::vector<size_t> goff(nf), foff(nf), poff(nf), roff(nf);
     std::vector<long long> key;
@@ -235,10 +233,6 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             L.s1_off += (int)s1_n;
             lv.push_back(L);
         }
-        for (int k = 0; k < P.nl1tiles; ++k)
-            order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
-        for (size_t k = P.nl1tiles; k < P.l1_order.size(); ++k)
-            s1order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
         roff[fi] = resp_n;
         resp_n += P.resp_total;
         s1_n += P.s1_total;
@@ -261,7 +255,6 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             return 1;
     }
     lv_begin[nf] = (int)lv.size();
-    order.insert(order.end(), s1order.begin(), s1order.end());
     const int nl = (int)lv.size();
     if (nl == 0) {
         // no level to run in any frame: no candidates
@@ -285,7 +278,8 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     if ((!device_ptr && grow(B.d_frames, B.frames_cap, frames_n) < 0) || grow(B.d_pre, B.pre_cap, pre_n) < 0 ||
         grow(B.d_gray, B.gray_cap, gray_n) < 0 || grow(B.d_s1, B.s1_cap, s1_n) < 0 ||
         grow(B.d_resp, B.resp_cap, resp_n) < 0 || grow(B.d_levels, B.levels_cap, (size_t)nl) < 0 ||
-        grow(B.d_l1_order, B.l1_cap, order.size()) < 0 || grow(B.d_counts, B.cnt_cap, (size_t)nb + 1) < 0)
+        grow(B.d_l1_order, B.l1_cap, (size_t)(l1tiles + tiles)) < 0 ||
+        grow(B.d_counts, B.cnt_cap, (size_t)nb + 1) < 0)
         return -1;
     if (moved) {
         // the offsets buffer follows the counts buffer's capacity
@@ -431,7 +425,19 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
     };
     const bool fresh = key != B.key;
     if (fresh) {
-        // the level table and tile order of this layout
+        // the level table and the tile tables of this layout (built only here: a replayed
+        // layout keeps them on the device): every frame's L1 tiles {level, tile} in
+        // XCD-band order, then every frame's stage-1 tiles in launch order
+        std::vector<int2> order;
+        order.reserve((size_t)(l1tiles + tiles));
+        for (int pass = 0; pass < 2; ++pass)
+            for (int fi = 0; fi < nf; ++fi) {
+                const Plan& P = *seg[fi].plan;
+                const size_t k0 = pass == 0 ? 0 : (size_t)P.nl1tiles;
+                const size_t k1 = pass == 0 ? (size_t)P.nl1tiles : P.l1_order.size();
+                for (size_t k = k0; k < k1; ++k)
+                    order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
+            }
         HIP_OK(hipMemcpyAsync(B.d_levels, lv.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, B.st));
         if (!order.empty())
             HIP_OK(hipMemcpyAsync(B.d_l1_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
