@@ -187,7 +187,11 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
 
     // layout of the batch: each frame's gray image, level table, L1-sum planes, response
     // maps and stage-1 tiles follow the previous frame's
-    std::vector<LevelDesc> lv;
+    // each frame's first level, tile, L1 tile and L1-sum offset in the batch (the level
+    // table itself, the frames' levels with these offsets, is built when the layout changes)
+    std::vector<int> tile_base(nf), l1_base(nf);
+    std::vector<size_t> s1_base(nf);
+    int nlv = 0;
     std::vector<int> lv_begin(nf + 1, 0), gpitch(nf);
     std::vector<size_t> goff(nf), foff(nf), poff(nf), roff(nf);
     std::vector<long long> key;
@@ -220,19 +224,11 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             poff[fi] = pre_n;
             pre_n += round_up((size_t)f.gw * f.gh, 256);
         }
-        lv_begin[fi] = (int)lv.size();
-        for (LevelDesc L : P.levels) {
-            L.frame = fi;
-            L.gp = gp;
-            L.gw = f.gw;
-            L.gh = f.gh;
-            L.gray_off = (long long)goff[fi];
-            L.resp_off += (int)resp_n;
-            L.tile_off += (int)tiles;
-            L.l1_tile_off += (int)l1tiles;
-            L.s1_off += (int)s1_n;
-            lv.push_back(L);
-        }
+        lv_begin[fi] = nlv;
+        nlv += (int)P.levels.size();
+        tile_base[fi] = (int)tiles;
+        l1_base[fi] = (int)l1tiles;
+        s1_base[fi] = s1_n;
         roff[fi] = resp_n;
         resp_n += P.resp_total;
         s1_n += P.s1_total;
@@ -254,8 +250,8 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             quads > INT_MAX)
             return 1;
     }
-    lv_begin[nf] = (int)lv.size();
-    const int nl = (int)lv.size();
+    lv_begin[nf] = nlv;
+    const int nl = nlv;
     if (nl == 0) {
         // no level to run in any frame: no candidates
         for (const BatchFrame& f : seg) faces[f.idx].clear();
@@ -438,6 +434,23 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
                 for (size_t k = k0; k < k1; ++k)
                     order.push_back(make_int2(P.l1_order[k].x + lv_begin[fi], P.l1_order[k].y));
             }
+        std::vector<LevelDesc> lv;
+        lv.reserve((size_t)nl);
+        for (int fi = 0; fi < nf; ++fi) {
+            const BatchFrame& f = seg[fi];
+            for (LevelDesc L : f.plan->levels) {
+                L.frame = fi;
+                L.gp = gpitch[fi];
+                L.gw = f.gw;
+                L.gh = f.gh;
+                L.gray_off = (long long)goff[fi];
+                L.resp_off += (int)roff[fi];
+                L.tile_off += tile_base[fi];
+                L.l1_tile_off += l1_base[fi];
+                L.s1_off += (int)s1_base[fi];
+                lv.push_back(L);
+            }
+        }
         HIP_OK(hipMemcpyAsync(B.d_levels, lv.data(), sizeof(LevelDesc) * nl, hipMemcpyHostToDevice, B.st));
         if (!order.empty())
             HIP_OK(hipMemcpyAsync(B.d_l1_order, order.data(), sizeof(int2) * order.size(), hipMemcpyHostToDevice,
@@ -518,11 +531,12 @@ int Detector::run_batch(const std::vector<BatchFrame>& seg, bool device_ptr, std
             lc[k] = C[c0 + k];
             lc[k].level -= l0;
             if (checks)
-                lr[k] = check_decision(P, lv[C[c0 + k].level].scale, S2 + 2 * (size_t)(c0 + k),
+                lr[k] = check_decision(P, f.plan->levels[lc[k].level].scale, S2 + 2 * (size_t)(c0 + k),
                                        S3 + 2 * (size_t)(c0 + k));
         }
-        const std::vector<Det> rects = replay(lv.data() + l0, l1 - l0, lc.data(), checks ? lr.data() : nullptr, cnt,
-                                              checks, st);
+        // the replay reads the levels' scales, which the frame's plan holds
+        const std::vector<Det> rects = replay(f.plan->levels.data(), l1 - l0, lc.data(), checks ? lr.data() : nullptr,
+                                              cnt, checks, st);
         emit_faces(rects, f.pre, faces[f.idx]);
     };
     if (!pool_) pool_.reset(new ReplayPool(std::max(0, std::min(host_cpus(), 16) - 1)));
